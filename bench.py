@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: WCSPH particle-steps/s on one MI355X.
+
+Workload (default): the 1,048,576-particle column scene (128^3 jittered
+lattice at dx = 0.01, h = 0.02, in a 2.56 x 1.28 x 2.56 box with a
+128 x 64 x 128 grid, dt = 1e-4, one substep per step).  A step is one full
+WCSPH substep (hash, sort, cell table, density, normals, forces + integrate,
+XSPH) with the state resident in HBM.  Prints one JSON line on stdout;
+progress goes to stderr.
+
+Roofline figure: the dominant kernel (forces fused with the integrator),
+algorithmic bytes 124 B per particle (reads pos, vel, normal, acc 16 B each,
+density/pressure/sound speed 12 B; writes pos, vel, acc 16 B each) divided by
+its mean duration from HIP events on the scene's stream over the timed
+region, against the 8 TB/s HBM peak.
+
+cpu_baseline: the FP64 CPU oracle (OpenMP build) timed on a bounded prefix of
+the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M particles; batched-scene steps/s + p99 step latency"
+HBM_PEAK = 8.0e12
+FORCE_BYTES_PER_PARTICLE = 124  # fused forces + integrator, see module docstring
+STEP_BYTES_PER_PARTICLE = 436   # compulsory bytes of the whole substep, plus 12 B per cell
+REF_PARTICLE_STEPS = 7.7e7      # README.md:2 of the reference: 1M particles at 77 FPS (RTX 2080), 1 substep/frame
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample duration")
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, target_s: float) -> dict:
+    from oracle.pyoracle import Oracle
+
+    orc = Oracle(omp=True)
+    p = cfg.pos.astype(np.float64)
+    v = cfg.vel.astype(np.float64)
+    a = np.zeros_like(p)
+    dt = float(np.float32(cfg.dt))
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        orc.step(cfg.params, dt, p, v, a, 0, want=())
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= target_s or steps >= cfg.steps:
+            break
+        if el / steps * (steps + 1) > 2.0 * target_s:
+            break
+    rate = cfg.n * steps / el
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": rate,
+        "unit": "particle-steps/s",
+        "cores": orc.threads,
+        "kind": "port",
+        "sample": f"FP64 oracle (OpenMP, {orc.threads} threads, {cpu}) on the first {steps} substeps of "
+                  f"{cfg.name} ({cfg.n} particles), {el:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the single-scene path is not sharded across GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    import libfluid_amd as lf
+    from libfluid_amd import _lib as L
+    from libfluid_amd import configs
+
+    cfg = configs.CONFIGS[args.config]()
+    n = cfg.n
+    log(f"[bench] {cfg.name}: {n} particles, grid {list(cfg.params.grid)}, warmup {args.warmup}, steps {args.steps}")
+    sc = lf.Scene(cfg.params, n)
+    sc.upload(cfg.pos, cfg.vel)
+
+    for _ in range(args.warmup):
+        sc.step(cfg.dt, 1)
+    sc.sync()
+    log("[bench] warmup done")
+
+    # Timed region: K steps; the force kernel is bracketed by HIP events.
+    sc.profile(["forces"])
+    sc.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sc.step(cfg.dt, 1)
+    sc.sync()
+    t1 = time.perf_counter()
+    prof = sc.profile_read()
+    sc.profile([])
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    f_launches, f_ms = prof["forces"]
+    f_avg_s = f_ms / max(f_launches, 1) / 1e3
+    achieved = FORCE_BYTES_PER_PARTICLE * n / f_avg_s
+    log(f"[bench] timed: {ms_per_step:.4f} ms/step, {value:.4e} particle-steps/s; forces {1e3 * f_avg_s:.4f} ms")
+
+    # Per-kernel breakdown (separate, all kernels bracketed).
+    sc.profile(list(L.KERNELS))
+    nb = min(50, args.steps)
+    for _ in range(nb):
+        sc.step(cfg.dt, 1)
+    sc.sync()
+    per = sc.profile_read()
+    sc.profile([])
+    kernels_ms = {k: (ms / c if c else None) for k, (c, ms) in per.items()}
+
+    # Per-step latency (each step synchronised).
+    lat = []
+    for _ in range(args.latency_steps):
+        s0 = time.perf_counter()
+        sc.step(cfg.dt, 1)
+        sc.sync()
+        lat.append(1e3 * (time.perf_counter() - s0))
+    sc.close()
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"forces_{cfg.name}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / REF_PARTICLE_STEPS,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{cfg.name}: {n} particles, jittered lattice dx=0.01, h=0.02, "
+                        f"box {[round(float(b), 4) for b in cfg.params.box]}, grid {list(cfg.params.grid)}, "
+                        f"dt=1e-4, 1 substep/step, plain WCSPH (K=2500, gamma=7, XSPH 0.5)",
+            "particles": n,
+            "cells": cfg.params.ncells,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "kernel": "k_forces (forces + leapfrog integrator)",
+            "bytes_per_particle": FORCE_BYTES_PER_PARTICLE,
+            "avg_ms": 1e3 * f_avg_s,
+            "launches": f_launches,
+        },
+        "step_roofline_frac": (STEP_BYTES_PER_PARTICLE + 12.0 * cfg.params.ncells / n) * value / HBM_PEAK,
+        "kernels_ms": kernels_ms,
+        "p50_step_ms": float(np.percentile(lat, 50)),
+        "p99_step_ms": float(np.percentile(lat, 99)),
+        "vs_baseline_ref": "README.md:2 of the reference, 1M particles at 77 FPS on an RTX 2080 = 7.7e7 particle-steps/s",
+    }
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

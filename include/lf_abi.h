@@ -1,0 +1,182 @@
+/*
+ * lf_abi.h -- C ABI of libfluid_hip.so, the MI355X (gfx950) WCSPH timestep.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.
+ * Every function returns LF_OK (0) or a negative LF_ERR_* code; the message
+ * of the last failure on the calling thread is returned by lf_last_error().
+ *
+ * The reference (NCCA/libfluid) has no C ABI: its boundary is the C++ class
+ * FluidSystem (libfluid/include/fluidsystem.h:16-126).  Each entry point
+ * below names the reference member it replaces; include/fluidsystem.h
+ * re-creates that class on top of these functions.
+ */
+#ifndef LF_ABI_H
+#define LF_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LF_ABI_VERSION 1
+
+/* Cell key of a particle outside the grid (NULL_HASH,
+ * common/include/fluiddefines.h:12). */
+#define LF_NULL_HASH 0xFFFFFFFFu
+
+enum lf_status {
+    LF_OK = 0,
+    LF_ERR_INVALID = -1,  /* bad argument */
+    LF_ERR_HIP = -2,      /* HIP runtime failure */
+    LF_ERR_ALLOC = -3,    /* device or host allocation failed */
+    LF_ERR_STATE = -4,    /* call not valid in the scene's current state */
+    LF_ERR_OVERFLOW = -5  /* a cell held more particles than the in-cell ranking supports */
+};
+
+/* Particle order for lf_download. */
+enum lf_order {
+    LF_ORDER_CELL = 0, /* current cell-sorted order, as exportToData copies it (fluidsystem.cu:168-172) */
+    LF_ORDER_ID = 1    /* particle-id order (the order particles were uploaded in) */
+};
+
+/*
+ * Scene parameters.  Same quantities as FluidParamData
+ * (libfluid/cudasrc/fluidparams.cuh:16-69) with the unit cube generalised:
+ * the fluid lives in the box [0,box], the uniform grid has grid[a] cells of
+ * size 1/inv_cell[a] along axis a, and the smoothing length h is independent
+ * of the grid (the reference ties h = 1/res, fluidparams.cu:62).  Cells must
+ * not be smaller than h.  Derived constants (h powers, kernel coefficients)
+ * are computed by the library.
+ */
+typedef struct lf_params {
+    float rest_density;    /* rho0 */
+    float gas_constant;    /* K: Tait stiffness */
+    float gamma;           /* Tait exponent */
+    float mass;            /* per-particle mass */
+    float particle_size;   /* boundary clamp margin (particle radius) */
+    float xsph_scale;      /* XSPH blend */
+    float surface_tension; /* sigma */
+    float viscosity;       /* nu */
+    float adhesion;        /* alpha */
+    float vel_limit;
+    float acc_limit;
+    float vdamp;           /* boundary velocity damping */
+    float gravity[3];
+    float h;               /* smoothing length */
+    float inv_cell[3];     /* 1 / cell size per axis (the reference's res) */
+    uint32_t grid[3];      /* cells per axis */
+    float box[3];          /* box extents; boundary SDF box is [0,box] */
+} lf_params;
+
+typedef struct lf_scene lf_scene;
+
+/* ---- library -------------------------------------------------------- */
+int lf_version(void);
+const char *lf_last_error(void);
+/* Number of visible HIP devices (0 when no GPU; never initialises a context). */
+int lf_device_count(int *count);
+
+/* ---- parameters ------------------------------------------------------ */
+/* Defaults of FluidSystem::init (fluidsystem.cu:196-235) on the unit cube with
+ * a res^3 grid and h = 1/res (FluidParams::setRes, fluidparams.cu:51-89). */
+int lf_params_default(lf_params *p, uint32_t res);
+/* Validates p (positive h and cells, cells >= h, finite coefficients). */
+int lf_params_check(const lf_params *p);
+
+/* ---- scene lifetime (FluidSystem::init / clear / ~FluidSystem,
+ *      fluidsystem.cu:18-52,196-265) ------------------------------------ */
+int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out);
+int lf_destroy(lf_scene *s);
+int lf_num_particles(const lf_scene *s, uint32_t *n);
+
+/* Setters of FluidSystem (fluidsystem.cu:412-475) collapse into one call;
+ * the grid (inv_cell, grid, box) cannot change after lf_create. */
+int lf_set_params(lf_scene *s, const lf_params *p);
+int lf_get_params(const lf_scene *s, lf_params *p);
+
+/* ---- data in / out -------------------------------------------------- */
+/* Writes particles with ids [first, first+count) from host float3 arrays
+ * (x,y,z interleaved).  vel/acc may be NULL (zero).  Replaces the per-point
+ * writes of createFluidBlock (fluidsystem.cu:121) and the copies of
+ * setupFromData (fluidsystem.cu:187-188). */
+int lf_upload(lf_scene *s, uint32_t first, uint32_t count, const float *pos_xyz, const float *vel_xyz,
+              const float *acc_xyz);
+/* Copies the state to host float3 arrays (any pointer may be NULL), in the
+ * given lf_order.  colour is DensityColourOperator of the last density
+ * (fluidutil.cu:67-80).  Replaces exportToData (fluidsystem.cu:154-173). */
+int lf_download(lf_scene *s, int order, float *pos_xyz, float *colour_rgb, float *vel_xyz, float *acc_xyz,
+                uint32_t *ids);
+
+/* ---- stepping (FluidSystem::advance, fluidsystem.cu:273-399) ---------- */
+/* Enqueues `substeps` substeps of dt/substeps on the scene's HIP stream and
+ * returns without waiting. */
+int lf_step(lf_scene *s, float dt, uint32_t substeps);
+/* Waits for the scene's stream and reports any deferred device error. */
+int lf_sync(lf_scene *s);
+/* Number of substeps run so far. */
+int lf_step_count(const lf_scene *s, uint64_t *count);
+
+/* ---- inspection of the last substep (parity tests, tooling) ---------- */
+enum lf_field {
+    LF_FIELD_KEYS = 0,        /* uint32[n]: cell key of each particle, previous order (NULL_HASH outside) */
+    LF_FIELD_START = 1,       /* uint32[ncells+2]: cell start table; [ncells] = first out-of-grid slot */
+    LF_FIELD_SORTED_IDS = 2,  /* uint32[n]: particle id in each slot of the new cell order */
+    LF_FIELD_DENSITY = 3,     /* float4[n]: (density, pressure, sound speed, 1/density^2), cell order */
+    LF_FIELD_NORMAL = 4,      /* float4[n]: (normal xyz, 0), cell order */
+    LF_FIELD_FORCES = 5,      /* float4[4n]: pressure, viscosity, tension, adhesion per particle
+                                 (written only with lf_set_debug(s,1)), cell order */
+    LF_FIELD_VEL_HALF = 6,    /* float4[n]: velocity after integration, before XSPH, cell order */
+    LF_FIELD_SORTED_POS = 7   /* float4[n]: pre-integration positions in cell order */
+};
+/* Copies `bytes` bytes of a raw device field into dst (synchronous). */
+int lf_get_field(lf_scene *s, int field, void *dst, size_t bytes);
+/* 1 = also write the four force components (LF_FIELD_FORCES) each substep. */
+int lf_set_debug(lf_scene *s, int on);
+/* Neighbour sets of the density pass for the current state (same FP32
+ * predicate as the kernels), by particle id: counts[n], and when list is not
+ * NULL the ids of each set, ascending, concatenated in id order.
+ * *total receives the number of pairs; cap bounds `list`. */
+int lf_neighbours(lf_scene *s, uint32_t *counts, uint32_t *list, uint64_t cap, uint64_t *total);
+
+/* ---- timing ------------------------------------------------------------ */
+/* Kernel classes timed by lf_profile_*. */
+enum lf_kernel {
+    LF_K_HASH = 0, LF_K_SCAN = 1, LF_K_SCATTER = 2, LF_K_GATHER = 3,
+    LF_K_DENSITY = 4, LF_K_NORMALS = 5, LF_K_FORCES = 6, LF_K_XSPH = 7,
+    LF_K_COUNT = 8
+};
+/* While on, every kernel launch of lf_step is bracketed by HIP events on the
+ * scene's stream.  lf_profile_read waits, then reports per kernel class the
+ * launch count and the summed event time in milliseconds, and resets. */
+int lf_profile_enable(lf_scene *s, int on);
+int lf_profile_read(lf_scene *s, uint64_t *launches, double *ms);
+/* Plain device-to-device copy kernel over `bytes` bytes (multiple of 16) of
+ * scene-owned scratch; used to calibrate hardware byte counters.  Returns
+ * the event-timed duration in ms. */
+int lf_copy_probe(lf_scene *s, size_t bytes, int repeats, double *ms);
+
+/* ---- host-side helpers (no device needed) ---------------------------- */
+/* createFluidBlock (fluidsystem.cu:66-127): lattice of up to n points in the
+ * box [mn, mx], x outermost and z innermost, FP32 float accumulators as in
+ * the reference.  Writes the points, returns the count written, the mass
+ * (particle volume x rho0) and the particle size (half the spacing). */
+int lf_fluid_block(uint32_t n, const float mn[3], const float mx[3], float rest_density, float *pos_xyz,
+                   uint32_t *written, float *mass, float *particle_size);
+/* Jittered lattice: nx*ny*nz points at origin + (i+0.5)*dx (x outermost,
+ * z innermost), each coordinate displaced by U(-jitter, jitter) drawn from
+ * SplitMix64(seed) in point order. */
+int lf_jittered_lattice(uint32_t nx, uint32_t ny, uint32_t nz, float dx, const float origin[3], float jitter,
+                        uint64_t seed, float *pos_xyz);
+/* n points uniform in [lo, hi) from SplitMix64(seed). */
+int lf_uniform_points(uint32_t n, const float lo[3], const float hi[3], uint64_t seed, float *pos_xyz);
+/* Houdini PGEOMETRY V5 ASCII writer, byte-compatible with the reference
+ * driver's dumpToGeo (test/src/main.cpp:10-74); writes "<base>.<cnt:04d>.geo". */
+int lf_write_geo(const char *filebase, uint32_t cnt, const float *pos_xyz, const float *colour_rgb,
+                 const float *vel_xyz, uint32_t n, float pscale);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LF_ABI_H */

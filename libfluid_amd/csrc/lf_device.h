@@ -1,0 +1,138 @@
+// lf_device.h -- per-scene kernel constants and device helpers shared by the
+// WCSPH kernels.  Included by HIP translation units only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lf {
+
+constexpr int kBlock = 256;             // threads per workgroup for every particle kernel
+constexpr uint32_t kRankFixMax = 4096;  // largest bucket the in-cell id ranking handles
+constexpr int kScanPerThread = 16;      // cell-count scan: entries per thread
+constexpr int kScanTile = kBlock * kScanPerThread;
+
+// Kernel-argument block: every constant a substep needs, derived on the host
+// (in double, rounded once to float) from lf_params.  Passed by value, so a
+// launch reads it through the kernarg segment (scalar loads).
+struct Consts {
+    // grid (FluidParamData m_res/m_res2/m_numCells, generalised per axis)
+    float inv_cell[3];
+    int gx, gy, gz;
+    uint32_t ncells;
+    float box[3];
+    // smoothing length
+    float h, h2, inv_h, eps;
+    // density + EOS
+    float mC6;        // m * 315/(64 pi h^9)
+    float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
+    float bd_rho0;    // rho0 * 0.0039062 (boundary density polynomial scale)
+    // normals
+    float hmCg6;      // h * m * (-945/(32 pi h^9))
+    // forces
+    float mCgs;       // m * (-45/(pi h^6))
+    float nu, sigma, alpha, m2, sigma_m, visc_eps, halfh, coh_off, quarter_inv_psize;
+    int has_visc, has_tension, has_adhesion;
+    // integrator
+    float g[3], m, dt, dtdt, vlim, vlim2, alim, alim2, vdamp1, psize;
+    // XSPH
+    float xsph_k;     // xsph * 2 m * C6
+};
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// Squared distance with a fixed FMA order.  The cell-key and pair-predicate
+// decisions are made on this exact expression, which the oracle reproduces.
+__device__ __forceinline__ float dist2(float4 a, float4 b) {
+    float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Cell coordinate along one axis: floor(x * res) (fluidutil.cu:15-18).
+__device__ __forceinline__ float cell_coord(float x, float inv_cell) { return floorf(x * inv_cell); }
+
+// Bucket of a position: cell key kx + gx*(ky + gy*kz) (fluidutil.cu:23-39), or
+// ncells for positions outside the grid (also NaN).
+__device__ __forceinline__ uint32_t bucket_of(const Consts &c, float4 p) {
+    float fx = cell_coord(p.x, c.inv_cell[0]);
+    float fy = cell_coord(p.y, c.inv_cell[1]);
+    float fz = cell_coord(p.z, c.inv_cell[2]);
+    bool in = fx >= 0.0f && fx < (float)c.gx && fy >= 0.0f && fy < (float)c.gy && fz >= 0.0f && fz < (float)c.gz;
+    if (!in) return c.ncells;
+    return (uint32_t)fx + (uint32_t)c.gx * ((uint32_t)fy + (uint32_t)c.gy * (uint32_t)fz);
+}
+
+__device__ __forceinline__ void cell_xyz(const Consts &c, float4 p, int &cx, int &cy, int &cz) {
+    cx = (int)cell_coord(p.x, c.inv_cell[0]);
+    cy = (int)cell_coord(p.y, c.inv_cell[1]);
+    cz = (int)cell_coord(p.z, c.inv_cell[2]);
+}
+
+// Visits every slot j of the clipped 3x3x3 cell stencil around (cx,cy,cz).
+// With keys ordered x fastest, the three cells of one x-row are contiguous in
+// the sorted arrays, so the stencil is nine contiguous slot ranges.
+template <class F>
+__device__ __forceinline__ void for_stencil(const Consts &c, const uint32_t *__restrict__ start, int cx, int cy,
+                                            int cz, F &&f) {
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < c.gx - 1 ? cx + 1 : c.gx - 1;
+    const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < c.gy - 1 ? cy + 1 : c.gy - 1;
+    const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < c.gz - 1 ? cz + 1 : c.gz - 1;
+    for (int z = z0; z <= z1; ++z) {
+        for (int y = y0; y <= y1; ++y) {
+            const uint32_t row = (uint32_t)(z * c.gy + y) * (uint32_t)c.gx;
+            const uint32_t b = start[row + (uint32_t)x0];
+            const uint32_t e = start[row + (uint32_t)x1 + 1u];
+            for (uint32_t j = b; j < e; ++j) f(j);
+        }
+    }
+}
+
+// ---- box boundary [0, box] (fluidutil.cu:85-112) ---------------------------
+// The reference forms q = |x - b| - b with b = box/2; -min(x, box - x) is the
+// same value and, near a wall, is exact in FP32.
+__device__ __forceinline__ float sdf_of_q(float qx, float qy, float qz) {
+    float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+    float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f), oz = fmaxf(qz, 0.0f);
+    float o2 = __builtin_fmaf(oz, oz, __builtin_fmaf(oy, oy, ox * ox));
+    return inside + (o2 > 0.0f ? sqrtf(o2) : 0.0f);
+}
+
+__device__ __forceinline__ float box_sdf(const Consts &c, float x, float y, float z) {
+    return sdf_of_q(-fminf(x, c.box[0] - x), -fminf(y, c.box[1] - y), -fminf(z, c.box[2] - z));
+}
+
+// boundaryNormalSDF: normalised central difference of the SDF with step 1e-4.
+__device__ __forceinline__ float3 box_normal(const Consts &c, float x, float y, float z) {
+    const float e = 0.0001f;
+    const float lx = x, ly = y, lz = z;
+    const float hx = c.box[0] - x, hy = c.box[1] - y, hz = c.box[2] - z;
+    const float qx = -fminf(lx, hx), qy = -fminf(ly, hy), qz = -fminf(lz, hz);
+    float nx = sdf_of_q(-fminf(lx + e, hx - e), qy, qz) - sdf_of_q(-fminf(lx - e, hx + e), qy, qz);
+    float ny = sdf_of_q(qx, -fminf(ly + e, hy - e), qz) - sdf_of_q(qx, -fminf(ly - e, hy + e), qz);
+    float nz = sdf_of_q(qx, qy, -fminf(lz + e, hz - e)) - sdf_of_q(qx, qy, -fminf(lz - e, hz + e));
+    float inv = 1.0f / sqrtf(__builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
+    return make_float3(nx * inv, ny * inv, nz * inv);
+}
+
+// DensityColourOperator (fluidutil.cu:67-80).
+__device__ __forceinline__ float3 density_colour(const Consts &c, float rho) {
+    float r = rho * c.inv_rho0;
+    if (r < 1.0f) return make_float3(0.0f, r, 1.0f - r);
+    if (r < 2.0f) return make_float3(r - 1.0f, 2.0f - r, 0.0f);
+    return make_float3(1.0f, 0.0f, 0.0f);
+}
+
+// Fast reciprocal / reciprocal square root (v_rcp_f32 / v_rsq_f32, ~1 ulp).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Workgroup index remapped so that each XCD (which takes workgroups
+// round-robin) processes one contiguous range of particles: neighbours of a
+// particle then mostly live in the same XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+}  // namespace lf

@@ -1,0 +1,585 @@
+// lf_kernels.hip -- CDNA4 (gfx950) kernels of one WCSPH substep.
+//
+// Layout: every per-particle array is SoA float4 (16 B, one dwordx4 load):
+//   P = (x, y, z, id bits)   V = (vx, vy, vz, 0)   A = (ax, ay, az, 0)
+//   D = (density, pressure, sound speed, 1/density)   N = (normal, 0)
+// The sorted arrays are in (cell key, particle id) order; `start` holds the
+// first slot of every bucket (ncells grid cells + one out-of-grid bucket)
+// plus an end sentinel.
+//
+// SPH here is a gather-bound stencil, not a dense contraction: the kernels
+// are VALU/L1 work over a 27-cell neighbour walk and never use MFMA.
+
+#include "lf_device.h"
+#include "lf_kernels.h"
+
+namespace lf {
+
+// ---------------------------------------------------------------------------
+// 1. hash + count (PointHashOperator, hash.cu:10-27; fill fluidsystem.cu:304).
+// The returned value of the count atomic is the particle's provisional rank
+// inside its bucket.  Counts are zeroed by the previous substep's scan.
+__global__ __launch_bounds__(kBlock) void k_hash(Consts c, const float4 *__restrict__ P, uint32_t n,
+                                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ ranks,
+                                                 uint32_t *__restrict__ counts) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = bucket_of(c, P[i]);
+    keys[i] = b;
+    ranks[i] = atomicAdd(&counts[b], 1u);
+}
+
+// ---------------------------------------------------------------------------
+// 2. cell start table: exclusive scan of the bucket counts
+// (thrust::exclusive_scan, fluidsystem.cu:316), three phases.
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, uint32_t &total) {
+    const uint32_t incl = wave_incl_scan(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 63) sh[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kBlock / 64; ++k) {
+        const uint32_t t = sh[k];
+        off += (k < w) ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + incl - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_sum(const uint32_t *__restrict__ counts, uint32_t m,
+                                                     uint32_t *__restrict__ bsum) {
+    __shared__ uint32_t sh[kBlock / 64];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) s += (base + k < m) ? counts[base + k] : 0u;
+    uint32_t tot;
+    (void)block_excl_scan(s, sh, tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the per-tile sums; writes the grand total
+// (the particle count) as the end sentinel start[m].
+__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *__restrict__ bsum, uint32_t nb,
+                                                     uint32_t *__restrict__ start, uint32_t m) {
+    __shared__ uint32_t sh[kBlock / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += kBlock) {
+        const uint32_t idx = base + threadIdx.x;
+        const uint32_t v = idx < nb ? bsum[idx] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, sh, tot);
+        if (idx < nb) bsum[idx] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) start[m] = carry;
+}
+
+// Local scan of each tile plus its offset; clears the counts for the next
+// substep.
+__global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ counts, uint32_t m,
+                                                      const uint32_t *__restrict__ bsum,
+                                                      uint32_t *__restrict__ start) {
+    __shared__ uint32_t sh[kBlock / 64];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
+    uint32_t v[kScanPerThread];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        v[k] = (base + k < m) ? counts[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = bsum[blockIdx.x] + block_excl_scan(s, sh, tot);
+#pragma unroll
+    for (int k = 0; k < kScanPerThread; ++k) {
+        if (base + k < m) {
+            start[base + k] = run;
+            counts[base + k] = 0u;
+        }
+        run += v[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. provisional scatter: slot = bucket start + atomic rank.
+__global__ __launch_bounds__(kBlock) void k_scatter(const float4 *__restrict__ P, const uint32_t *__restrict__ keys,
+                                                    const uint32_t *__restrict__ ranks,
+                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    uint32_t *__restrict__ slot_src, uint32_t *__restrict__ slot_id) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = start[keys[i]] + ranks[i];
+    slot_src[s] = i;
+    slot_id[s] = __float_as_uint(P[i].w);
+}
+
+// 4. stable placement + payload gather (sort_by_key of Points, Velocity,
+// Acceleration, fluidsystem.cu:309-313).  The final slot of a particle is
+// its bucket start plus the number of bucket members with a smaller id, so
+// the order is (key, id) whatever order the atomics handed out.
+__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ slot_src,
+                                                   const uint32_t *__restrict__ slot_id,
+                                                   const uint32_t *__restrict__ keys,
+                                                   const uint32_t *__restrict__ start, const float4 *__restrict__ P,
+                                                   const float4 *__restrict__ V, const float4 *__restrict__ A,
+                                                   uint32_t n, float4 *__restrict__ Ps, float4 *__restrict__ Vs,
+                                                   float4 *__restrict__ As, uint32_t *__restrict__ err) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t src = slot_src[s], id = slot_id[s];
+    const uint32_t b = keys[src];
+    const uint32_t lo = start[b], hi = start[b + 1];
+    uint32_t f;
+    if (hi - lo <= kRankFixMax) {
+        uint32_t r = 0;
+        for (uint32_t t = lo; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
+        f = lo + r;
+    } else {
+        f = s;  // bucket too large to rank: keep the provisional slot, report it
+        if (s == lo) atomicOr(err, 1u);
+    }
+    Ps[f] = P[src];
+    Vs[f] = V[src];
+    As[f] = A[src];
+}
+
+// ---------------------------------------------------------------------------
+// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84).
+__global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__restrict__ Ps,
+                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    float4 *__restrict__ D) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i >= start[c.ncells]) {  // out-of-grid bucket
+        D[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 pi = Ps[i];
+    int cx, cy, cz;
+    cell_xyz(c, pi, cx, cy, cz);
+    float S = 0.0f;
+    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+        const float rr = dist2(pi, Ps[j]);
+        if (rr <= c.h2 && j != i) {
+            const float t = c.h2 - rr;
+            S += t * t * t;
+        }
+    });
+    S *= c.mC6;  // sum of m * poly6
+    // Boundary density term rho0 * (1 - boundaryDensity(d)) for d < h
+    // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), written in t = d/h.
+    const float dist = fabsf(box_sdf(c, pi.x, pi.y, pi.z));
+    float corrected = S;
+    if (dist < c.h) {
+        const float t = dist * c.inv_h, t2 = t * t;
+        const float q = 128.0f - 315.0f * t + t * t2 * (420.0f - 378.0f * t2 + t2 * t2 * (180.0f - 35.0f * t2));
+        corrected += c.bd_rho0 * q;
+    }
+    const float rho = (S == 0.0f) ? c.rho0 : corrected;
+    const float ratio = corrected * c.inv_rho0;
+    const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (powf(ratio, c.gamma) - 1.0f);
+    const float cs = c.K * powf(ratio, c.gamma_m1);
+    D[i] = make_float4(rho, press, cs, 1.0f / rho);
+}
+
+// 6. surface normals (computeNormals, fluidphysics.cu:154-210):
+// n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
+// gradPoly6's factor r.
+__global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Ps,
+                                                    const float4 *__restrict__ D,
+                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    float4 *__restrict__ N) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (i >= start[c.ncells]) {
+        N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
+    const float4 pi = Ps[i];
+    int cx, cy, cz;
+    cell_xyz(c, pi, cx, cy, cz);
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+        const float4 q = Ps[j];
+        const float rr = dist2(pi, q);
+        if (rr <= c.h2 && rr > c.eps && j != i) {
+            const float4 dj = D[j];
+            if (dj.x > c.eps) {
+                const float t = c.h2 - rr;
+                const float w = t * t * dj.w;
+                sx += w * (pi.x - q.x);
+                sy += w * (pi.y - q.y);
+                sz += w * (pi.z - q.z);
+            }
+        }
+    });
+    N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
+}
+
+// 7. forces (computeAllForces, fluidphysics.cu:215-352) fused with the
+// leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
+// fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
+// still integrated.  Fdbg (may be null) receives the four force components.
+__global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
+                                                   const float4 *__restrict__ Vs, const float4 *__restrict__ As,
+                                                   const float4 *__restrict__ D, const float4 *__restrict__ N,
+                                                   const uint32_t *__restrict__ start, uint32_t n,
+                                                   float4 *__restrict__ Pout, float4 *__restrict__ Vout,
+                                                   float4 *__restrict__ Aout, float4 *__restrict__ Fdbg) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
+    float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
+    if (i < start[c.ncells]) {
+        const float4 di = D[i], ni = N[i];
+        const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
+        const float iri2 = iri * iri;
+        float Px = 0.f, Py = 0.f, Pz = 0.f;     // pressure sum
+        float Qx = 0.f, Qy = 0.f, Qz = 0.f;     // artificial-viscosity part of the viscosity sum
+        float Tx = 0.f, Ty = 0.f, Tz = 0.f;     // cohesion + curvature sum
+        int cx, cy, cz;
+        cell_xyz(c, pi, cx, cy, cz);
+        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+            const float4 q = Ps[j];
+            const float rr = dist2(pi, q);
+            if (!(rr <= c.h2 && rr > c.eps && j != i)) return;
+            const float4 dj = D[j];
+            if (!(dj.x > c.eps)) return;
+            const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
+            const float inv_r = frsq(rr);
+            const float r = rr * inv_r;
+            const float hr = fmaxf(c.h - r, 0.0f);
+            const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
+            const float irj = dj.w;
+            const float mid = dj.y * iri2 + p_i * (irj * irj);
+            const float cp = cm * mid;
+            Px += cp * rx;
+            Py += cp * ry;
+            Pz += cp * rz;
+            if (c.has_visc) {
+                const float4 vj = Vs[j];
+                const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
+                if (ux < 0.0f) {
+                    const float mu = c.h * ux * frcp(rr + c.visc_eps);
+                    const float Pi = -c.nu * (cs_i + dj.z) * mu * frcp(rho_i + dj.x);
+                    const float cv = cm * Pi;
+                    Qx += cv * rx;
+                    Qy += cv * ry;
+                    Qz += cv * rz;
+                }
+            }
+            if (c.has_tension) {
+                const float4 nj = N[j];
+                const float hr3r3 = hr * hr * hr * r * r * r;
+                const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
+                const float ch = -c.m2 * coh * inv_r;
+                Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
+                Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
+                Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
+            }
+        });
+        fP = make_float3(Px, Py, Pz);
+        fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
+        // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
+        const float ts = c.sigma * c.rho0 * iri;
+        fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
+        if (c.has_adhesion) {
+            const float z = -fminf(box_sdf(c, pi.x, pi.y, pi.z), 0.0f) * c.quarter_inv_psize;
+            if (z < 1.0f) {
+                const float3 nb = box_normal(c, pi.x, pi.y, pi.z);
+                const float oz = 1.0f - z;
+                const float s = c.alpha * rho_i * c.inv_rho0 / (oz * oz * oz);
+                fA = make_float3(s * nb.x, s * nb.y, s * nb.z);
+            }
+        }
+    }
+    if (Fdbg) {
+        Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
+        Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
+        Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
+        Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
+    }
+    // Leapfrog: a' = F m + g (clamped); x += v dt + a dt^2/2; v += (a + a') dt/2.
+    const float Fx = fP.x + fT.x + fA.x + fV.x, Fy = fP.y + fT.y + fA.y + fV.y, Fz = fP.z + fT.z + fA.z + fV.z;
+    float ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
+    float s2 = ax * ax + ay * ay + az * az;
+    if (s2 > c.alim2) {
+        const float k = c.alim / sqrtf(s2);
+        ax *= k;
+        ay *= k;
+        az *= k;
+    }
+    float x = pi.x + (vi.x * c.dt + 0.5f * ai.x * c.dtdt);
+    float y = pi.y + (vi.y * c.dt + 0.5f * ai.y * c.dtdt);
+    float z = pi.z + (vi.z * c.dt + 0.5f * ai.z * c.dtdt);
+    float vx = vi.x + 0.5f * (ai.x + ax) * c.dt;
+    float vy = vi.y + 0.5f * (ai.y + ay) * c.dt;
+    float vz = vi.z + 0.5f * (ai.z + az) * c.dt;
+    s2 = vx * vx + vy * vy + vz * vz;
+    if (s2 > c.vlim2) {
+        const float k = c.vlim / sqrtf(s2);
+        vx *= k;
+        vy *= k;
+        vz *= k;
+    }
+    const float dd = box_sdf(c, x, y, z);
+    if (dd > -c.psize) {
+        const float3 nb = box_normal(c, x, y, z);
+        const float push = dd + c.psize;
+        x -= push * nb.x;
+        y -= push * nb.y;
+        z -= push * nb.z;
+        const float vn = c.vdamp1 * (nb.x * vx + nb.y * vy + nb.z * vz);
+        vx -= vn * nb.x;
+        vy -= vn * nb.y;
+        vz -= vn * nb.z;
+    }
+    Pout[i] = make_float4(x, y, z, pi.w);
+    Vout[i] = make_float4(vx, vy, vz, 0.0f);
+    Aout[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
+// the stencil of the pre-integration cell, integrated positions, and the
+// pre-XSPH velocity of every neighbour (Jacobi update into Vout).
+__global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restrict__ Ps,
+                                                 const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
+                                                 const float4 *__restrict__ D, const uint32_t *__restrict__ start,
+                                                 uint32_t n, float4 *__restrict__ Vout) {
+    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 vi = Vh[i];
+    if (i >= start[c.ncells]) {
+        Vout[i] = vi;
+        return;
+    }
+    const float4 xi = Pn[i];
+    const float rho_i = D[i].x;
+    int cx, cy, cz;
+    cell_xyz(c, Ps[i], cx, cy, cz);
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+        const float rr = dist2(xi, Pn[j]);
+        if (rr <= c.h2 && j != i) {
+            const float sr = rho_i + D[j].x;
+            if (sr > c.eps) {
+                const float t = c.h2 - rr;
+                const float w = t * t * t * frcp(sr);
+                const float4 vj = Vh[j];
+                sx += w * (vj.x - vi.x);
+                sy += w * (vj.y - vi.y);
+                sz += w * (vj.z - vi.z);
+            }
+        }
+    });
+    Vout[i] = make_float4(vi.x + c.xsph_k * sx, vi.y + c.xsph_k * sy, vi.z + c.xsph_k * sz, 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Data movement and inspection helpers.
+
+__global__ __launch_bounds__(kBlock) void k_upload(const float *__restrict__ stage, uint32_t first, uint32_t count,
+                                                   float4 *__restrict__ P, float4 *__restrict__ V,
+                                                   float4 *__restrict__ A) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const float *s = stage + 9 * (size_t)k;
+    const uint32_t id = first + k;
+    P[id] = make_float4(s[0], s[1], s[2], __uint_as_float(id));
+    V[id] = make_float4(s[3], s[4], s[5], 0.0f);
+    A[id] = make_float4(s[6], s[7], s[8], 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_init_state(uint32_t n, float4 *__restrict__ P, float4 *__restrict__ V,
+                                                       float4 *__restrict__ A, float4 *__restrict__ D) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= n) return;
+    P[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(k));
+    V[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    D[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Moves slot s to slot id(s): state back to particle-id order.
+__global__ __launch_bounds__(kBlock) void k_to_id_order(const float4 *__restrict__ P, const float4 *__restrict__ V,
+                                                        const float4 *__restrict__ A, const float4 *__restrict__ D,
+                                                        uint32_t n, float4 *__restrict__ P2, float4 *__restrict__ V2,
+                                                        float4 *__restrict__ A2, float4 *__restrict__ D2) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = P[s];
+    const uint32_t id = __float_as_uint(p.w);
+    P2[id] = p;
+    V2[id] = V[s];
+    A2[id] = A[s];
+    D2[id] = D[s];
+}
+
+// AoS staging for downloads: 12 floats per particle (pos, colour, vel, acc).
+__global__ __launch_bounds__(kBlock) void k_export(Consts c, const float4 *__restrict__ P,
+                                                   const float4 *__restrict__ V, const float4 *__restrict__ A,
+                                                   const float4 *__restrict__ D, uint32_t n, int by_id,
+                                                   float *__restrict__ out, uint32_t *__restrict__ ids) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    const float4 p = P[s], v = V[s], a = A[s];
+    const uint32_t id = __float_as_uint(p.w);
+    const uint32_t dst = by_id ? id : s;
+    const float3 col = density_colour(c, D[s].x);
+    float *o = out + 12 * (size_t)dst;
+    o[0] = p.x; o[1] = p.y; o[2] = p.z;
+    o[3] = col.x; o[4] = col.y; o[5] = col.z;
+    o[6] = v.x; o[7] = v.y; o[8] = v.z;
+    o[9] = a.x; o[10] = a.y; o[11] = a.z;
+    ids[dst] = id;
+}
+
+// Neighbour sets of the density pass (debug): count, then fill.
+__global__ __launch_bounds__(kBlock) void k_nb_count(Consts c, const float4 *__restrict__ Ps,
+                                                     const uint32_t *__restrict__ start, uint32_t n,
+                                                     uint32_t *__restrict__ cnt) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = 0;
+    if (i < start[c.ncells]) {
+        const float4 pi = Ps[i];
+        int cx, cy, cz;
+        cell_xyz(c, pi, cx, cy, cz);
+        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+            if (dist2(pi, Ps[j]) <= c.h2 && j != i) ++k;
+        });
+    }
+    cnt[i] = k;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_fill(Consts c, const float4 *__restrict__ Ps,
+                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    const uint64_t *__restrict__ off, uint32_t *__restrict__ list) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n || i >= start[c.ncells]) return;
+    const float4 pi = Ps[i];
+    int cx, cy, cz;
+    cell_xyz(c, pi, cx, cy, cz);
+    uint64_t o = off[i];
+    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+        const float4 q = Ps[j];
+        if (dist2(pi, q) <= c.h2 && j != i) list[o++] = __float_as_uint(q.w);
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                 size_t n4) {
+    size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (; k < n4; k += stride) dst[k] = src[k];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers.
+
+static inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+void launch_hash(hipStream_t st, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
+                 uint32_t *counts) {
+    if (n) hipLaunchKernelGGL(k_hash, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, P, n, keys, ranks, counts);
+}
+
+uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
+
+void launch_scan(hipStream_t st, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
+    const uint32_t nb = scan_tiles(m);
+    hipLaunchKernelGGL(k_scan_sum, dim3(nb), dim3(kBlock), 0, st, counts, m, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, st, bsum, nb, start, m);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, counts, m, bsum, start);
+}
+
+void launch_scatter(hipStream_t st, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
+                    const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id) {
+    if (n)
+        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, keys, ranks, start, n, slot_src,
+                           slot_id);
+}
+
+void launch_gather(hipStream_t st, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
+                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, uint32_t n, float4 *Ps,
+                   float4 *Vs, float4 *As, uint32_t *err) {
+    if (n)
+        hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, slot_src, slot_id, keys, start, P, V,
+                           A, n, Ps, Vs, As, err);
+}
+
+void launch_density(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
+                    float4 *D) {
+    if (n) hipLaunchKernelGGL(k_density, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, D);
+}
+
+void launch_normals(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
+                    uint32_t n, float4 *N) {
+    if (n) hipLaunchKernelGGL(k_normals, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, D, start, n, N);
+}
+
+void launch_forces(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
+                   const float4 *D, const float4 *N, const uint32_t *start, uint32_t n, float4 *Pout, float4 *Vout,
+                   float4 *Aout, float4 *Fdbg) {
+    if (n)
+        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Vs, As, D, N, start, n, Pout,
+                           Vout, Aout, Fdbg);
+}
+
+void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
+                 const float4 *D, const uint32_t *start, uint32_t n, float4 *Vout) {
+    if (n) hipLaunchKernelGGL(k_xsph, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Pn, Vh, D, start, n, Vout);
+}
+
+void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
+                   float4 *A) {
+    if (count) hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A);
+}
+
+void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *D) {
+    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, D);
+}
+
+void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *D,
+                        uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *D2) {
+    if (n)
+        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, D, n, P2, V2, A2, D2);
+}
+
+void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
+                   const float4 *D, uint32_t n, int by_id, float *out, uint32_t *ids) {
+    if (n) hipLaunchKernelGGL(k_export, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, P, V, A, D, n, by_id, out, ids);
+}
+
+void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
+                     uint32_t *cnt) {
+    if (n) hipLaunchKernelGGL(k_nb_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, cnt);
+}
+
+void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
+                    const uint64_t *off, uint32_t *list) {
+    if (n) hipLaunchKernelGGL(k_nb_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, off, list);
+}
+
+void launch_copy(hipStream_t st, const float4 *src, float4 *dst, size_t n4) {
+    const size_t want = (n4 + kBlock - 1) / kBlock;
+    const unsigned nb = (unsigned)(want < 65536 ? (want ? want : 1) : 65536);
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, st, src, dst, n4);
+}
+
+}  // namespace lf

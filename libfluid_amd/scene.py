@@ -1,0 +1,187 @@
+"""Python handle over one lf_scene (device-resident WCSPH state).
+
+Thin: every method is one C-ABI call; arrays cross as numpy float32 (N,3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import LfParams, check, fptr, lib, uptr
+
+
+def default_params(res: int) -> LfParams:
+    """FluidSystem::init defaults on the unit cube with a res^3 grid."""
+    p = LfParams()
+    check(lib.lf_params_default(C.byref(p), int(res)), "lf_params_default")
+    return p
+
+
+def make_params(**kw) -> LfParams:
+    """Params from keyword values on top of the res=1 defaults."""
+    p = default_params(1)
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        cur = getattr(p, k)
+        if hasattr(cur, "__len__"):
+            for i, x in enumerate(v):
+                cur[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Scene:
+    """One WCSPH scene on the current HIP device."""
+
+    def __init__(self, params: LfParams, num_particles: int):
+        self._h = C.c_void_p()
+        check(lib.lf_create(C.byref(params), int(num_particles), C.byref(self._h)), "lf_create")
+        self.n = int(num_particles)
+        self.params = LfParams.from_buffer_copy(params)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if self._h and self._h.value:
+            lib.lf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- parameters ---------------------------------------------------------
+    def set_params(self, params: LfParams) -> None:
+        check(lib.lf_set_params(self._h, C.byref(params)), "lf_set_params")
+        self.params = LfParams.from_buffer_copy(params)
+
+    # -- data ---------------------------------------------------------------
+    def upload(self, pos: np.ndarray, vel: np.ndarray | None = None, acc: np.ndarray | None = None,
+               first: int = 0) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        acc = None if acc is None else np.ascontiguousarray(acc, dtype=np.float32).reshape(-1, 3)
+        check(lib.lf_upload(self._h, int(first), pos.shape[0], fptr(pos), fptr(vel), fptr(acc)), "lf_upload")
+
+    def download(self, order: int = L.LF_ORDER_ID) -> dict:
+        n = self.n
+        out = {k: np.empty((n, 3), np.float32) for k in ("pos", "colour", "vel", "acc")}
+        ids = np.empty(n, np.uint32)
+        check(lib.lf_download(self._h, int(order), fptr(out["pos"]), fptr(out["colour"]), fptr(out["vel"]),
+                              fptr(out["acc"]), uptr(ids)), "lf_download")
+        out["ids"] = ids
+        return out
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, dt: float, substeps: int = 1) -> None:
+        check(lib.lf_step(self._h, C.c_float(dt), int(substeps)), "lf_step")
+
+    def sync(self) -> None:
+        check(lib.lf_sync(self._h), "lf_sync")
+
+    def step_count(self) -> int:
+        c = C.c_uint64(0)
+        check(lib.lf_step_count(self._h, C.byref(c)), "lf_step_count")
+        return c.value
+
+    # -- inspection ---------------------------------------------------------
+    def set_debug(self, on: bool) -> None:
+        check(lib.lf_set_debug(self._h, 1 if on else 0), "lf_set_debug")
+
+    def field(self, which: int) -> np.ndarray:
+        n, nc = self.n, self.params.ncells
+        shapes = {
+            L.FIELD_KEYS: (np.uint32, (n,)),
+            L.FIELD_START: (np.uint32, (nc + 2,)),
+            L.FIELD_SORTED_IDS: (np.uint32, (n,)),
+            L.FIELD_DENSITY: (np.float32, (n, 4)),
+            L.FIELD_NORMAL: (np.float32, (n, 4)),
+            L.FIELD_FORCES: (np.float32, (n, 4, 4)),
+            L.FIELD_VEL_HALF: (np.float32, (n, 4)),
+            L.FIELD_SORTED_POS: (np.float32, (n, 4)),
+        }
+        dt, shape = shapes[which]
+        a = np.empty(shape, dt)
+        check(lib.lf_get_field(self._h, int(which), a.ctypes.data_as(C.c_void_p), a.nbytes), "lf_get_field")
+        return a
+
+    def neighbours(self):
+        """(counts by id, concatenated ascending neighbour ids) of the density pass."""
+        counts = np.empty(self.n, np.uint32)
+        total = C.c_uint64(0)
+        check(lib.lf_neighbours(self._h, uptr(counts), None, 0, C.byref(total)), "lf_neighbours")
+        lst = np.empty(max(total.value, 1), np.uint32)
+        check(lib.lf_neighbours(self._h, uptr(counts), uptr(lst), lst.size, C.byref(total)), "lf_neighbours")
+        return counts, lst[: total.value]
+
+    # -- timing -------------------------------------------------------------
+    def profile(self, kernels=()) -> None:
+        mask = 0
+        for k in kernels:
+            mask |= 1 << L.K_INDEX[k]
+        check(lib.lf_profile_enable(self._h, mask), "lf_profile_enable")
+
+    def profile_read(self) -> dict:
+        cnt = (C.c_uint64 * len(L.KERNELS))()
+        ms = (C.c_double * len(L.KERNELS))()
+        check(lib.lf_profile_read(self._h, cnt, ms), "lf_profile_read")
+        return {k: (int(cnt[i]), float(ms[i])) for i, k in enumerate(L.KERNELS)}
+
+    def copy_probe(self, nbytes: int, repeats: int = 10) -> float:
+        ms = C.c_double(0.0)
+        check(lib.lf_copy_probe(self._h, int(nbytes), int(repeats), C.byref(ms)), "lf_copy_probe")
+        return ms.value
+
+
+# -- host-side generators (no device needed) ----------------------------------
+
+def fluid_block(n: int, mn, mx, rest_density: float):
+    """createFluidBlock: (points[written,3], mass, particle_size)."""
+    pts = np.zeros((n, 3), np.float32)
+    w = C.c_uint32(0)
+    m = C.c_float(0)
+    ps = C.c_float(0)
+    a = np.asarray(mn, np.float32)
+    b = np.asarray(mx, np.float32)
+    check(lib.lf_fluid_block(int(n), fptr(a), fptr(b), C.c_float(rest_density), fptr(pts), C.byref(w),
+                             C.byref(m), C.byref(ps)), "lf_fluid_block")
+    return pts[: w.value].copy(), m.value, ps.value
+
+
+def jittered_lattice(nx: int, ny: int, nz: int, dx: float, origin, jitter: float, seed: int) -> np.ndarray:
+    out = np.empty((nx * ny * nz, 3), np.float32)
+    o = np.asarray(origin, np.float32)
+    check(lib.lf_jittered_lattice(nx, ny, nz, C.c_float(dx), fptr(o), C.c_float(jitter), C.c_uint64(seed),
+                                  fptr(out)), "lf_jittered_lattice")
+    return out
+
+
+def uniform_points(n: int, lo, hi, seed: int) -> np.ndarray:
+    out = np.empty((n, 3), np.float32)
+    a = np.asarray(lo, np.float32)
+    b = np.asarray(hi, np.float32)
+    check(lib.lf_uniform_points(int(n), fptr(a), fptr(b), C.c_uint64(seed), fptr(out)), "lf_uniform_points")
+    return out
+
+
+def write_geo(filebase: str, cnt: int, pos, colour, vel, pscale: float) -> None:
+    pos = np.ascontiguousarray(pos, np.float32)
+    colour = np.ascontiguousarray(colour, np.float32)
+    vel = np.ascontiguousarray(vel, np.float32)
+    check(lib.lf_write_geo(filebase.encode(), int(cnt), fptr(pos), fptr(colour), fptr(vel), pos.shape[0],
+                           C.c_float(pscale)), "lf_write_geo")
