@@ -10,8 +10,9 @@ seeded generators in the library, so every consumer sees identical bytes.
 
   dambreak_c1   32^3 jittered lattice (x,y,z in [0.005, 0.315]) in a
                 1.0 x 0.5 x 0.5 box, grid 50 x 25 x 25; 32,768 particles
-  column_c2     128^3 jittered lattice (a 1.28 m column) in the corner of a
-                2.56 x 1.28 x 2.56 box, grid 128 x 64 x 128; 1,048,576 particles
+  column_c2     128 x 64 x 128 jittered lattice (1.28 x 0.64 x 1.28 m) in the
+                corner of a 2.56 x 1.28 x 2.56 box (4x its footprint), grid
+                128 x 64 x 128; exactly 1,048,576 particles
   spray_c3      256 x 115 x 256 jittered block plus 851,968 uniform spray
                 particles in y in [1.5, 2.5) over a 2.56^3 box, grid 128^3;
                 8,388,608 particles
@@ -79,7 +80,7 @@ def dambreak_c1(seed: int = 1) -> Config:
 
 def column_c2(seed: int = 2) -> Config:
     p = _plain_params((2.56, 1.28, 2.56), (128, 64, 128))
-    pos = jittered_lattice(128, 128, 128, DX, (0.0, 0.0, 0.0), 0.05 * DX, seed)
+    pos = jittered_lattice(128, 64, 128, DX, (0.0, 0.0, 0.0), 0.05 * DX, seed)
     return Config("column_c2", p, pos, np.zeros_like(pos), 1e-4, 1000)
 
 

@@ -20,21 +20,22 @@ struct Consts {
     float inv_cell[3];
     int gx, gy, gz;
     uint32_t ncells;
-    float box[3];
+    double half_box[3];
     // smoothing length
     float h, h2, inv_h, eps;
     // density + EOS
     float mC6;        // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
-    float bd_rho0;    // rho0 * 0.0039062 (boundary density polynomial scale)
     // normals
     float hmCg6;      // h * m * (-945/(32 pi h^9))
     // forces
     float mCgs;       // m * (-45/(pi h^6))
-    float nu, sigma, alpha, m2, sigma_m, visc_eps, halfh, coh_off, quarter_inv_psize;
+    float nu, sigma, alpha, m2, sigma_m, visc_eps, halfh, coh_off;
+    double quarter_inv_psize, alpha_inv_rho0;
     int has_visc, has_tension, has_adhesion;
     // integrator
-    float g[3], m, dt, dtdt, vlim, vlim2, alim, alim2, vdamp1, psize;
+    double g[3], m, dt, dtdt, vlim, vlim2, alim, alim2;
+    double vdamp1, psize, h_d, bd_rho0_d;
     // XSPH
     float xsph_k;     // xsph * 2 m * C6
 };
@@ -88,30 +89,30 @@ __device__ __forceinline__ void for_stencil(const Consts &c, const uint32_t *__r
 }
 
 // ---- box boundary [0, box] (fluidutil.cu:85-112) ---------------------------
-// The reference forms q = |x - b| - b with b = box/2; -min(x, box - x) is the
-// same value and, near a wall, is exact in FP32.
-__device__ __forceinline__ float sdf_of_q(float qx, float qy, float qz) {
-    float inside = fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
-    float ox = fmaxf(qx, 0.0f), oy = fmaxf(qy, 0.0f), oz = fmaxf(qz, 0.0f);
-    float o2 = __builtin_fmaf(oz, oz, __builtin_fmaf(oy, oy, ox * ox));
-    return inside + (o2 > 0.0f ? sqrtf(o2) : 0.0f);
+// Per-particle boundary quantities are evaluated in FP64 on the FP32
+// position: the box SDF and its finite-difference normal are ill-conditioned
+// near edges and at the adhesion singularity, and FP64 costs nothing at one
+// evaluation per particle.  Same formula as the reference: q = |x - b| - b
+// with b the box half extent, sdf = min(max(q), 0) + |max(q, 0)|.
+__device__ __forceinline__ double box_sdf(const Consts &c, double x, double y, double z) {
+    const double qx = fabs(x - c.half_box[0]) - c.half_box[0];
+    const double qy = fabs(y - c.half_box[1]) - c.half_box[1];
+    const double qz = fabs(z - c.half_box[2]) - c.half_box[2];
+    const double inside = fmin(fmax(qx, fmax(qy, qz)), 0.0);
+    const double ox = fmax(qx, 0.0), oy = fmax(qy, 0.0), oz = fmax(qz, 0.0);
+    return inside + sqrt(ox * ox + oy * oy + oz * oz);
 }
 
-__device__ __forceinline__ float box_sdf(const Consts &c, float x, float y, float z) {
-    return sdf_of_q(-fminf(x, c.box[0] - x), -fminf(y, c.box[1] - y), -fminf(z, c.box[2] - z));
-}
-
-// boundaryNormalSDF: normalised central difference of the SDF with step 1e-4.
-__device__ __forceinline__ float3 box_normal(const Consts &c, float x, float y, float z) {
-    const float e = 0.0001f;
-    const float lx = x, ly = y, lz = z;
-    const float hx = c.box[0] - x, hy = c.box[1] - y, hz = c.box[2] - z;
-    const float qx = -fminf(lx, hx), qy = -fminf(ly, hy), qz = -fminf(lz, hz);
-    float nx = sdf_of_q(-fminf(lx + e, hx - e), qy, qz) - sdf_of_q(-fminf(lx - e, hx + e), qy, qz);
-    float ny = sdf_of_q(qx, -fminf(ly + e, hy - e), qz) - sdf_of_q(qx, -fminf(ly - e, hy + e), qz);
-    float nz = sdf_of_q(qx, qy, -fminf(lz + e, hz - e)) - sdf_of_q(qx, qy, -fminf(lz - e, hz + e));
-    float inv = 1.0f / sqrtf(__builtin_fmaf(nz, nz, __builtin_fmaf(ny, ny, nx * nx)));
-    return make_float3(nx * inv, ny * inv, nz * inv);
+// boundaryNormalSDF: normalised central difference of the SDF, step 1e-4f.
+__device__ __forceinline__ void box_normal(const Consts &c, double x, double y, double z, double n[3]) {
+    const double e = (double)0.0001f;
+    const double nx = box_sdf(c, x + e, y, z) - box_sdf(c, x - e, y, z);
+    const double ny = box_sdf(c, x, y + e, z) - box_sdf(c, x, y - e, z);
+    const double nz = box_sdf(c, x, y, z + e) - box_sdf(c, x, y, z - e);
+    const double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz);
+    n[0] = nx * inv;
+    n[1] = ny * inv;
+    n[2] = nz * inv;
 }
 
 // DensityColourOperator (fluidutil.cu:67-80).

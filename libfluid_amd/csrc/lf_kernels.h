@@ -16,23 +16,23 @@ void launch_scan(hipStream_t st, uint32_t *counts, uint32_t m, uint32_t *bsum, u
 void launch_scatter(hipStream_t st, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id);
 void launch_gather(hipStream_t st, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, uint32_t n, float4 *Ps,
-                   float4 *Vs, float4 *As, uint32_t *err);
+                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+                   uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
 void launch_density(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
                     float4 *D);
 void launch_normals(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
                     uint32_t n, float4 *N);
 void launch_forces(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *D, const float4 *N, const uint32_t *start, uint32_t n, float4 *Pout, float4 *Vout,
-                   float4 *Aout, float4 *Fdbg);
+                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
+                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg);
 void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
                  const float4 *D, const uint32_t *start, uint32_t n, float4 *Vout);
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A);
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *D);
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *D,
-                        uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *D2);
+                   float4 *A, float4 *L);
+void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *L, float4 *D);
+void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+                        const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2);
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
                    const float4 *D, uint32_t n, int by_id, float *out, uint32_t *ids);
 void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,

@@ -2,6 +2,7 @@
 //
 // Layout: every per-particle array is SoA float4 (16 B, one dwordx4 load):
 //   P = (x, y, z, id bits)   V = (vx, vy, vz, 0)   A = (ax, ay, az, 0)
+//   L = low-order position words: the position is the double-float P.xyz + L.xyz
 //   D = (density, pressure, sound speed, 1/density)   N = (normal, 0)
 // The sorted arrays are in (cell key, particle id) order; `start` holds the
 // first slot of every bucket (ncells grid cells + one out-of-grid bucket)
@@ -17,16 +18,31 @@ namespace lf {
 
 // ---------------------------------------------------------------------------
 // 1. hash + count (PointHashOperator, hash.cu:10-27; fill fluidsystem.cu:304).
-// The returned value of the count atomic is the particle's provisional rank
-// inside its bucket.  Counts are zeroed by the previous substep's scan.
+// The value returned by the count atomic is the particle's provisional rank
+// inside its bucket.  Particles arrive in the previous cell order, so runs of
+// lanes share a bucket: each run issues one atomic for its length and hands
+// out consecutive ranks.  Counts are zeroed by the previous substep's scan.
 __global__ __launch_bounds__(kBlock) void k_hash(Consts c, const float4 *__restrict__ P, uint32_t n,
                                                  uint32_t *__restrict__ keys, uint32_t *__restrict__ ranks,
                                                  uint32_t *__restrict__ counts) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t b = bucket_of(c, P[i]);
-    keys[i] = b;
-    ranks[i] = atomicAdd(&counts[b], 1u);
+    const int lane = threadIdx.x & 63;
+    const bool active = i < n;
+    const uint32_t b = active ? bucket_of(c, P[i]) : 0xFFFFFFFFu;
+    const uint32_t prev = __shfl_up(b, 1, 64);
+    const bool head = lane == 0 || prev != b;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+    const int first = 63 - __clzll(heads & upto);
+    const uint64_t above = heads & ~upto;
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    uint32_t base = 0;
+    if (active && lane == first) base = atomicAdd(&counts[b], (uint32_t)(next - first));
+    base = __shfl(base, first, 64);
+    if (active) {
+        keys[i] = b;
+        ranks[i] = base + (uint32_t)(lane - first);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -137,8 +153,10 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
                                                    const uint32_t *__restrict__ keys,
                                                    const uint32_t *__restrict__ start, const float4 *__restrict__ P,
                                                    const float4 *__restrict__ V, const float4 *__restrict__ A,
-                                                   uint32_t n, float4 *__restrict__ Ps, float4 *__restrict__ Vs,
-                                                   float4 *__restrict__ As, uint32_t *__restrict__ err) {
+                                                   const float4 *__restrict__ L, uint32_t n,
+                                                   float4 *__restrict__ Ps, float4 *__restrict__ Vs,
+                                                   float4 *__restrict__ As, float4 *__restrict__ Ls,
+                                                   uint32_t *__restrict__ err) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint32_t src = slot_src[s], id = slot_id[s];
@@ -156,6 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
     Ps[f] = P[src];
     Vs[f] = V[src];
     As[f] = A[src];
+    Ls[f] = L[src];
 }
 
 // ---------------------------------------------------------------------------
@@ -181,14 +200,14 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
         }
     });
     S *= c.mC6;  // sum of m * poly6
-    // Boundary density term rho0 * (1 - boundaryDensity(d)) for d < h
-    // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), written in t = d/h.
-    const float dist = fabsf(box_sdf(c, pi.x, pi.y, pi.z));
+    // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
+    // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.
+    const double dist = fabs(box_sdf(c, pi.x, pi.y, pi.z));
     float corrected = S;
-    if (dist < c.h) {
-        const float t = dist * c.inv_h, t2 = t * t;
-        const float q = 128.0f - 315.0f * t + t * t2 * (420.0f - 378.0f * t2 + t2 * t2 * (180.0f - 35.0f * t2));
-        corrected += c.bd_rho0 * q;
+    if (dist < c.h_d) {
+        const double t = dist / c.h_d, t2 = t * t;
+        const double q = 128.0 - 315.0 * t + t * t2 * (420.0 - 378.0 * t2 + t2 * t2 * (180.0 - 35.0 * t2));
+        corrected = (float)((double)S + c.bd_rho0_d * q);
     }
     const float rho = (S == 0.0f) ? c.rho0 : corrected;
     const float ratio = corrected * c.inv_rho0;
@@ -237,10 +256,11 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // still integrated.  Fdbg (may be null) receives the four force components.
 __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
                                                    const float4 *__restrict__ Vs, const float4 *__restrict__ As,
-                                                   const float4 *__restrict__ D, const float4 *__restrict__ N,
-                                                   const uint32_t *__restrict__ start, uint32_t n,
-                                                   float4 *__restrict__ Pout, float4 *__restrict__ Vout,
-                                                   float4 *__restrict__ Aout, float4 *__restrict__ Fdbg) {
+                                                   const float4 *__restrict__ Ls, const float4 *__restrict__ D,
+                                                   const float4 *__restrict__ N, const uint32_t *__restrict__ start,
+                                                   uint32_t n, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
+                                                   float4 *__restrict__ Aout, float4 *__restrict__ Lout,
+                                                   float4 *__restrict__ Fdbg) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
@@ -299,12 +319,14 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
         const float ts = c.sigma * c.rho0 * iri;
         fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
         if (c.has_adhesion) {
-            const float z = -fminf(box_sdf(c, pi.x, pi.y, pi.z), 0.0f) * c.quarter_inv_psize;
-            if (z < 1.0f) {
-                const float3 nb = box_normal(c, pi.x, pi.y, pi.z);
-                const float oz = 1.0f - z;
-                const float s = c.alpha * rho_i * c.inv_rho0 / (oz * oz * oz);
-                fA = make_float3(s * nb.x, s * nb.y, s * nb.z);
+            // Adhesion toward the walls (fluidphysics.cu:330-350), FP64.
+            const double z = -fmin(box_sdf(c, pi.x, pi.y, pi.z), 0.0) * c.quarter_inv_psize;
+            if (z < 1.0) {
+                double nb[3];
+                box_normal(c, pi.x, pi.y, pi.z, nb);
+                const double oz = 1.0 - z;
+                const double s = c.alpha_inv_rho0 * (double)rho_i / (oz * oz * oz);
+                fA = make_float3((float)(s * nb[0]), (float)(s * nb[1]), (float)(s * nb[2]));
             }
         }
     }
@@ -314,44 +336,54 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
         Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
         Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
     }
-    // Leapfrog: a' = F m + g (clamped); x += v dt + a dt^2/2; v += (a + a') dt/2.
-    const float Fx = fP.x + fT.x + fA.x + fV.x, Fy = fP.y + fT.y + fA.y + fV.y, Fz = fP.z + fT.z + fA.z + fV.z;
-    float ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
-    float s2 = ax * ax + ay * ay + az * az;
+    // Leapfrog (fluidintegrator.cu:36-69), per particle in FP64 on the FP32
+    // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
+    // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
+    // the boundary projection and reflection.
+    const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
+    const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
+    const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
+    double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
+    double s2 = ax * ax + ay * ay + az * az;
     if (s2 > c.alim2) {
-        const float k = c.alim / sqrtf(s2);
+        const double k = c.alim / sqrt(s2);
         ax *= k;
         ay *= k;
         az *= k;
     }
-    float x = pi.x + (vi.x * c.dt + 0.5f * ai.x * c.dtdt);
-    float y = pi.y + (vi.y * c.dt + 0.5f * ai.y * c.dtdt);
-    float z = pi.z + (vi.z * c.dt + 0.5f * ai.z * c.dtdt);
-    float vx = vi.x + 0.5f * (ai.x + ax) * c.dt;
-    float vy = vi.y + 0.5f * (ai.y + ay) * c.dt;
-    float vz = vi.z + 0.5f * (ai.z + az) * c.dt;
+    const float4 li = Ls[i];
+    double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
+    double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
+    double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
+    double vx = (double)vi.x + 0.5 * ((double)ai.x + ax) * c.dt;
+    double vy = (double)vi.y + 0.5 * ((double)ai.y + ay) * c.dt;
+    double vz = (double)vi.z + 0.5 * ((double)ai.z + az) * c.dt;
     s2 = vx * vx + vy * vy + vz * vz;
     if (s2 > c.vlim2) {
-        const float k = c.vlim / sqrtf(s2);
+        const double k = c.vlim / sqrt(s2);
         vx *= k;
         vy *= k;
         vz *= k;
     }
-    const float dd = box_sdf(c, x, y, z);
+    // boundaryCheckSDF (fluidintegrator.cu:11-26).
+    const double dd = box_sdf(c, x, y, z);
     if (dd > -c.psize) {
-        const float3 nb = box_normal(c, x, y, z);
-        const float push = dd + c.psize;
-        x -= push * nb.x;
-        y -= push * nb.y;
-        z -= push * nb.z;
-        const float vn = c.vdamp1 * (nb.x * vx + nb.y * vy + nb.z * vz);
-        vx -= vn * nb.x;
-        vy -= vn * nb.y;
-        vz -= vn * nb.z;
+        double nb[3];
+        box_normal(c, x, y, z, nb);
+        const double push = dd + c.psize;
+        const double vn = c.vdamp1 * (nb[0] * vx + nb[1] * vy + nb[2] * vz);
+        x -= push * nb[0];
+        y -= push * nb[1];
+        z -= push * nb[2];
+        vx -= vn * nb[0];
+        vy -= vn * nb[1];
+        vz -= vn * nb[2];
     }
-    Pout[i] = make_float4(x, y, z, pi.w);
-    Vout[i] = make_float4(vx, vy, vz, 0.0f);
-    Aout[i] = make_float4(ax, ay, az, 0.0f);
+    const float hx = (float)x, hy = (float)y, hz = (float)z;
+    Pout[i] = make_float4(hx, hy, hz, pi.w);
+    Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
+    Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
+    Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
@@ -395,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restr
 
 __global__ __launch_bounds__(kBlock) void k_upload(const float *__restrict__ stage, uint32_t first, uint32_t count,
                                                    float4 *__restrict__ P, float4 *__restrict__ V,
-                                                   float4 *__restrict__ A) {
+                                                   float4 *__restrict__ A, float4 *__restrict__ L) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= count) return;
     const float *s = stage + 9 * (size_t)k;
@@ -403,23 +435,28 @@ __global__ __launch_bounds__(kBlock) void k_upload(const float *__restrict__ sta
     P[id] = make_float4(s[0], s[1], s[2], __uint_as_float(id));
     V[id] = make_float4(s[3], s[4], s[5], 0.0f);
     A[id] = make_float4(s[6], s[7], s[8], 0.0f);
+    L[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_init_state(uint32_t n, float4 *__restrict__ P, float4 *__restrict__ V,
-                                                       float4 *__restrict__ A, float4 *__restrict__ D) {
+                                                       float4 *__restrict__ A, float4 *__restrict__ L,
+                                                       float4 *__restrict__ D) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= n) return;
     P[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(k));
     V[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     A[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L[k] = make_float4(0.f, 0.f, 0.f, 0.f);
     D[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Moves slot s to slot id(s): state back to particle-id order.
 __global__ __launch_bounds__(kBlock) void k_to_id_order(const float4 *__restrict__ P, const float4 *__restrict__ V,
-                                                        const float4 *__restrict__ A, const float4 *__restrict__ D,
-                                                        uint32_t n, float4 *__restrict__ P2, float4 *__restrict__ V2,
-                                                        float4 *__restrict__ A2, float4 *__restrict__ D2) {
+                                                        const float4 *__restrict__ A, const float4 *__restrict__ L,
+                                                        const float4 *__restrict__ D, uint32_t n,
+                                                        float4 *__restrict__ P2, float4 *__restrict__ V2,
+                                                        float4 *__restrict__ A2, float4 *__restrict__ L2,
+                                                        float4 *__restrict__ D2) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const float4 p = P[s];
@@ -427,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_to_id_order(const float4 *__restrict
     P2[id] = p;
     V2[id] = V[s];
     A2[id] = A[s];
+    L2[id] = L[s];
     D2[id] = D[s];
 }
 
@@ -516,11 +554,11 @@ void launch_scatter(hipStream_t st, const float4 *P, const uint32_t *keys, const
 }
 
 void launch_gather(hipStream_t st, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, uint32_t n, float4 *Ps,
-                   float4 *Vs, float4 *As, uint32_t *err) {
+                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+                   uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err) {
     if (n)
         hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, slot_src, slot_id, keys, start, P, V,
-                           A, n, Ps, Vs, As, err);
+                           A, L, n, Ps, Vs, As, Ls, err);
 }
 
 void launch_density(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
@@ -534,11 +572,11 @@ void launch_normals(hipStream_t st, const Consts &c, const float4 *Ps, const flo
 }
 
 void launch_forces(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *D, const float4 *N, const uint32_t *start, uint32_t n, float4 *Pout, float4 *Vout,
-                   float4 *Aout, float4 *Fdbg) {
+                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
+                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg) {
     if (n)
-        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Vs, As, D, N, start, n, Pout,
-                           Vout, Aout, Fdbg);
+        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Vs, As, Ls, D, N, start, n,
+                           Pout, Vout, Aout, Lout, Fdbg);
 }
 
 void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
@@ -547,18 +585,20 @@ void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4
 }
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A) {
-    if (count) hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A);
+                   float4 *A, float4 *L) {
+    if (count)
+        hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A, L);
 }
 
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *D) {
-    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, D);
+void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *L, float4 *D) {
+    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, L, D);
 }
 
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *D,
-                        uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *D2) {
+void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+                        const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2) {
     if (n)
-        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, D, n, P2, V2, A2, D2);
+        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, L, D, n, P2, V2, A2, L2,
+                           D2);
 }
 
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,

@@ -50,6 +50,7 @@ struct lf_scene {
     uint64_t steps = 0;
     // state (P0,V0,A0) and the substep's working arrays
     float4 *P0 = nullptr, *P1 = nullptr, *V0 = nullptr, *V1 = nullptr, *V2 = nullptr, *A0 = nullptr, *A1 = nullptr;
+    float4 *L0 = nullptr, *L1 = nullptr;  // low-order position words
     float4 *D = nullptr, *N = nullptr, *F = nullptr;
     uint32_t *keys = nullptr, *ranks = nullptr, *slot_src = nullptr, *slot_id = nullptr;
     uint32_t *counts = nullptr, *start = nullptr, *bsum = nullptr, *err = nullptr;
@@ -74,7 +75,7 @@ int dalloc(T **p, size_t count) {
 }
 
 void free_all(lf_scene *s) {
-    void *bufs[] = {s->P0, s->P1, s->V0, s->V1, s->V2, s->A0, s->A1, s->D, s->N, s->F,
+    void *bufs[] = {s->P0, s->P1, s->V0, s->V1, s->V2, s->A0, s->A1, s->L0, s->L1, s->D, s->N, s->F,
                     s->keys, s->ranks, s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -94,8 +95,8 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt) {
     std::memset(&c, 0, sizeof(c));
     for (int a = 0; a < 3; ++a) {
         c.inv_cell[a] = p.inv_cell[a];
-        c.box[a] = p.box[a];
-        c.g[a] = p.gravity[a];
+        c.half_box[a] = 0.5 * (double)p.box[a];
+        c.g[a] = (double)p.gravity[a];
     }
     c.gx = (int)p.grid[0];
     c.gy = (int)p.grid[1];
@@ -119,7 +120,8 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt) {
     c.gamma = p.gamma;
     c.gamma_m1 = p.gamma - 1.0f;
     c.K = p.gas_constant;
-    c.bd_rho0 = (float)((double)p.rest_density * (double)0.0039062f);
+    c.bd_rho0_d = (double)p.rest_density * (double)0.0039062f;  // fluidkernel.cu:65
+    c.h_d = h;
     c.hmCg6 = (float)(h * m * Cg6);
     c.mCgs = (float)(m * Cgs);
     c.nu = p.viscosity;
@@ -130,18 +132,19 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt) {
     c.visc_eps = (float)(0.01 * (double)c.h2);
     c.halfh = 0.5f * p.h;
     c.coh_off = (float)(h6 * (double)0.0156f);
-    c.quarter_inv_psize = (float)(0.25 / (double)p.particle_size);
+    c.quarter_inv_psize = 0.25 / (double)p.particle_size;
+    c.alpha_inv_rho0 = (double)p.adhesion * inv_rho0;
     c.has_visc = p.viscosity != 0.0f;
     c.has_tension = p.surface_tension != 0.0f;
     c.has_adhesion = p.adhesion != 0.0f;
-    c.m = p.mass;
-    c.dt = dt;
-    c.dtdt = dt * dt;
-    c.vlim = p.vel_limit;
-    c.vlim2 = p.vel_limit * p.vel_limit;
-    c.alim = p.acc_limit;
-    c.alim2 = p.acc_limit * p.acc_limit;
-    c.vdamp1 = 1.0f + p.vdamp;
+    c.m = m;
+    c.dt = (double)dt;
+    c.dtdt = (double)dt * (double)dt;
+    c.vlim = (double)p.vel_limit;
+    c.vlim2 = c.vlim * c.vlim;
+    c.alim = (double)p.acc_limit;
+    c.alim2 = c.alim * c.alim;
+    c.vdamp1 = 1.0 + (double)p.vdamp;
     c.psize = p.particle_size;
     c.xsph_k = (float)((double)p.xsph_scale * 2.0 * m * C6);
     return c;
@@ -185,8 +188,8 @@ void enqueue_sort(lf_scene *s, const lf::Consts &c) {
     timed(s, LF_K_SCATTER,
           [&] { lf::launch_scatter(st, s->P0, s->keys, s->ranks, s->start, s->n, s->slot_src, s->slot_id); });
     timed(s, LF_K_GATHER, [&] {
-        lf::launch_gather(st, s->slot_src, s->slot_id, s->keys, s->start, s->P0, s->V0, s->A0, s->n, s->P1, s->V1,
-                          s->A1, s->err);
+        lf::launch_gather(st, s->slot_src, s->slot_id, s->keys, s->start, s->P0, s->V0, s->A0, s->L0, s->n, s->P1,
+                          s->V1, s->A1, s->L1, s->err);
     });
 }
 
@@ -198,8 +201,8 @@ void enqueue_substep(lf_scene *s, const lf::Consts &c) {
     timed(s, LF_K_DENSITY, [&] { lf::launch_density(st, c, s->P1, s->start, s->n, s->D); });
     timed(s, LF_K_NORMALS, [&] { lf::launch_normals(st, c, s->P1, s->D, s->start, s->n, s->N); });
     timed(s, LF_K_FORCES, [&] {
-        lf::launch_forces(st, c, s->P1, s->V1, s->A1, s->D, s->N, s->start, s->n, s->P0, s->V2, s->A0,
-                          s->debug ? s->F : nullptr);
+        lf::launch_forces(st, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->n, s->P0, s->V2, s->A0,
+                          s->L0, s->debug ? s->F : nullptr);
     });
     timed(s, LF_K_XSPH, [&] { lf::launch_xsph(st, c, s->P1, s->P0, s->V2, s->D, s->start, s->n, s->V0); });
     s->identity = false;
@@ -214,11 +217,12 @@ int check_launch(const char *what) {
 // Brings the state back to particle-id order (setup-time only).
 int to_id_order(lf_scene *s) {
     if (s->identity) return LF_OK;
-    lf::launch_to_id_order(s->stream, s->P0, s->V0, s->A0, s->D, s->n, s->P1, s->V1, s->A1, s->N);
+    lf::launch_to_id_order(s->stream, s->P0, s->V0, s->A0, s->L0, s->D, s->n, s->P1, s->V1, s->A1, s->L1, s->N);
     if (int rc = check_launch("to_id_order")) return rc;
     std::swap(s->P0, s->P1);
     std::swap(s->V0, s->V1);
     std::swap(s->A0, s->A1);
+    std::swap(s->L0, s->L1);
     std::swap(s->D, s->N);
     s->identity = true;
     return LF_OK;
@@ -260,6 +264,8 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     if (!rc) rc = dalloc(&s->V2, n);
     if (!rc) rc = dalloc(&s->A0, n);
     if (!rc) rc = dalloc(&s->A1, n);
+    if (!rc) rc = dalloc(&s->L0, n);
+    if (!rc) rc = dalloc(&s->L1, n);
     if (!rc) rc = dalloc(&s->D, n);
     if (!rc) rc = dalloc(&s->N, n);
     if (!rc) rc = dalloc(&s->keys, n);
@@ -278,7 +284,7 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
         if (e != hipSuccess) rc = fail(LF_ERR_HIP, "lf_create: %s", hipGetErrorString(e));
     }
     if (!rc) {
-        lf::launch_init_state(s->stream, s->n, s->P0, s->V0, s->A0, s->D);
+        lf::launch_init_state(s->stream, s->n, s->P0, s->V0, s->A0, s->L0, s->D);
         rc = check_launch("init_state");
     }
     if (!rc) {
@@ -342,7 +348,7 @@ int lf_upload(lf_scene *s, uint32_t first, uint32_t count, const float *pos, con
     if (int rc = dalloc(&d_stage, stage.size())) return rc;
     hipError_t e = hipMemcpyAsync(d_stage, stage.data(), stage.size() * sizeof(float), hipMemcpyHostToDevice, s->stream);
     if (e == hipSuccess) {
-        lf::launch_upload(s->stream, d_stage, first, count, s->P0, s->V0, s->A0);
+        lf::launch_upload(s->stream, d_stage, first, count, s->P0, s->V0, s->A0, s->L0);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

@@ -91,6 +91,11 @@ class Scene:
     def step(self, dt: float, substeps: int = 1) -> None:
         check(lib.lf_step(self._h, C.c_float(dt), int(substeps)), "lf_step")
 
+    def run(self, dt: float, steps: int) -> None:
+        """Enqueue `steps` substeps of length dt each (lf_step(dt, 1) per step)."""
+        for _ in range(int(steps)):
+            check(lib.lf_step(self._h, C.c_float(dt), 1), "lf_step")
+
     def sync(self) -> None:
         check(lib.lf_sync(self._h), "lf_sync")
 

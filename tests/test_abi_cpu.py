@@ -1,0 +1,169 @@
+"""CPU tests of the C ABI library (no GPU calls that compute).
+
+* libfluid_hip.so loads and exports every function include/lf_abi.h declares;
+* parameter defaults are those of FluidSystem::init (fluidsystem.cu:202-235);
+* parameter validation, scene-input generators, the .geo writer's bytes
+  (test/src/main.cpp:10-74) and the fluidtest driver's command line.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import libfluid_amd as lf
+from libfluid_amd import _lib as L
+from libfluid_amd.scene import default_params, fluid_block, jittered_lattice, uniform_points, write_geo
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "lf_abi.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) >= 20
+    lib = C.CDLL(L.LIB_PATH)
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    assert sorted(L.EXPORTED) == names  # the Python binding covers the whole ABI
+
+
+def test_abi_version_and_struct_layout(tmp_path):
+    """The ctypes mirror of lf_params matches the C header (size and offsets)."""
+    assert L.lib.lf_version() == 1
+    fields = [f for f, _ in L.LfParams._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lf_abi.h"\nint main(void){'
+                   + 'printf("%zu", sizeof(lf_params));'
+                   + "".join(f'printf(" %zu", offsetof(lf_params, {f}));' for f in fields) + "return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == C.sizeof(L.LfParams)
+    assert out[1:] == [getattr(L.LfParams, f).offset for f in fields]
+
+
+def test_params_default_match_reference_init():
+    p = default_params(16)
+    assert (p.rest_density, p.gas_constant, p.gamma, p.mass, p.particle_size) == (1000.0, 100.0, 7.0, 1.0, 1.0)
+    assert (p.vel_limit, p.acc_limit, p.xsph_scale, p.vdamp) == (5.0, 150.0, 0.5, 0.0)
+    # constructor defaults that init() leaves alone (fluidparams.cuh:81-97)
+    assert (p.surface_tension, p.viscosity, p.adhesion) == (1.0, np.float32(0.1), 1.0)
+    assert list(p.gravity) == [0.0, np.float32(-9.8), 0.0]
+    assert p.h == np.float32(1 / 16) and list(p.grid) == [16] * 3 and list(p.inv_cell) == [16.0] * 3
+    assert list(p.box) == [1.0] * 3
+
+
+@pytest.mark.parametrize("field,value", [("h", 0.0), ("h", 0.2), ("particle_size", 0.0), ("mass", -1.0),
+                                         ("gamma", float("nan"))])
+def test_params_check_rejects(field, value):
+    p = default_params(16)
+    setattr(p, field, value)
+    assert L.lib.lf_params_check(C.byref(p)) == L.LF_ERR_INVALID
+    assert L.lib.lf_last_error()
+
+
+def test_params_check_grid_must_cover_box():
+    p = default_params(16)
+    p.grid[1] = 8
+    assert L.lib.lf_params_check(C.byref(p)) == L.LF_ERR_INVALID
+    assert b"cover" in L.lib.lf_last_error()
+
+
+def test_create_without_device_fails_cleanly():
+    if lf.device_count() > 0:
+        pytest.skip("a device is visible")
+    h = C.c_void_p()
+    rc = L.lib.lf_create(C.byref(default_params(8)), 100, C.byref(h))
+    assert rc < 0 and not h.value and L.lib.lf_last_error()
+
+
+def test_fluid_block_lattice():
+    """createFluidBlock: spacing d = min(extent/round(extent/cbrt(V/n))), points
+    from min + d/2 in steps of d, x outermost, truncated at n."""
+    pts, mass, psize = fluid_block(1000, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), 1000.0)
+    d = np.float32(1.0) / (np.float32(1.0) / np.cbrt(np.float32(1e-3)) + np.float32(0.5))
+    assert psize == pytest.approx(0.5 * d, rel=1e-6)
+    assert mass == pytest.approx(1.0, rel=1e-6)  # (V / n) * rho0
+    assert pts.shape == (1000, 3)
+    np.testing.assert_allclose(pts[0], [0.5 * d] * 3, rtol=1e-6)
+    np.testing.assert_allclose(pts[1] - pts[0], [0, 0, d], rtol=1e-5)  # z innermost
+    assert np.all(pts <= 1.0) and np.all(pts >= 0.0)
+    # fewer lattice sites than requested: every site written once
+    pts2, _, _ = fluid_block(50, (0.0, 0.0, 0.0), (0.1, 0.1, 0.1), 1000.0)
+    assert len(np.unique(pts2, axis=0)) == pts2.shape[0]
+
+
+def test_jittered_lattice_seeded():
+    a = jittered_lattice(4, 5, 6, 0.01, (0.1, 0.2, 0.3), 0.0005, 42)
+    b = jittered_lattice(4, 5, 6, 0.01, (0.1, 0.2, 0.3), 0.0005, 42)
+    c = jittered_lattice(4, 5, 6, 0.01, (0.1, 0.2, 0.3), 0.0005, 43)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    i, j, k = np.meshgrid(np.arange(4), np.arange(5), np.arange(6), indexing="ij")
+    base = np.stack([0.1 + (i + 0.5) * 0.01, 0.2 + (j + 0.5) * 0.01, 0.3 + (k + 0.5) * 0.01], -1).reshape(-1, 3)
+    dev = np.abs(a - base)
+    assert dev.max() <= 0.0005 + 1e-7 and dev.max() > 0.0004
+
+
+def test_uniform_points_in_range():
+    p = uniform_points(10000, (0.0, 1.5, 0.0), (2.56, 2.5, 2.56), 7)
+    assert np.all(p >= [0.0, 1.5, 0.0]) and np.all(p < [2.56, 2.5, 2.56])
+    assert abs(p[:, 1].mean() - 2.0) < 0.02
+
+
+def _expected_geo(pos, col, vel, pscale):
+    """The byte layout of dumpToGeo (test/src/main.cpp:29-70); C++ default
+    float output is %g with 6 significant digits."""
+    g = lambda v: "%g" % float(v)  # noqa: E731
+    n = len(pos)
+    out = ["PGEOMETRY V5\n", f"NPoints {n} NPrims 1\n", "NPointGroups 0 NPrimGroups 1\n",
+           "NPointAttrib 3  NVertexAttrib 0 NPrimAttrib 2 NAttrib 0\n", "PointAttrib \n",
+           "Cd 3 float 1 1 1\n", "v 3 float 0 0 0\n", "pscale 1 float 1\n"]
+    for p, c, v in zip(pos, col, vel):
+        out.append(f"{g(p[0])} {g(p[1])} {g(p[2])} 1 ({g(c[0])} {g(c[1])} {g(c[2])} "
+                   f"{g(v[0])} {g(v[1])} {g(v[2])} {g(pscale)})\n")
+    out += ["PrimitiveAttrib\n", "generator 1 index 1 location1\n", "dopobject 1 index 1 /obj/AutoDopNetwork:1\n",
+            f"Part {n} " + "".join(f"{i} " for i in range(n)) + " [0\t0]\n", "box_object1 unordered\n", "1 1\n",
+            "beginExtra\n", "endExtra\n"]
+    return "".join(out)
+
+
+def test_geo_writer_bytes(tmp_path):
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-1, 1, (7, 3)).astype(np.float32)
+    col = rng.uniform(0, 1, (7, 3)).astype(np.float32)
+    vel = (rng.normal(0, 1e-3, (7, 3))).astype(np.float32)
+    vel[0] = [1e-7, -123456.0, 0.0]
+    base = str(tmp_path / "sim")
+    write_geo(base, 12, pos, col, vel, np.float32(0.0125))
+    text = open(base + ".0012.geo").read()
+    assert text == _expected_geo(pos, col, vel, np.float32(0.0125))
+
+
+def test_fluidtest_driver_cli():
+    exe = os.path.join(ROOT, "libfluid_amd", "bin", "fluidtest")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0
+    for flag in ("-n", "-g", "-a", "-v", "-s", "-d", "-t", "-p", "-q", "-f"):
+        assert f"  {flag}, --" in r.stdout
+    r = subprocess.run([exe, "-n", "100"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "filebase" in r.stderr
+
+
+def test_python_mirror_has_reference_api():
+    public = ["setup", "setupFromData", "exportToData", "getPscale", "advance", "setRestDensity",
+              "setParticleSize", "setGasConstant", "setLimits", "setXSPHScale", "setSurfaceTension",
+              "setViscosity", "setAdhesion", "setVDamp", "setGamma", "setMass"]
+    protected = ["init", "clear", "createFluidBlock"]
+    for name in public + protected:
+        assert callable(getattr(lf.FluidSystem, name)), name
+    header = open(os.path.join(ROOT, "include", "fluidsystem.h")).read()
+    for name in public + protected:
+        assert re.search(rf"\b{name}\s*\(", header), name
