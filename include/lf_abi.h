@@ -123,17 +123,22 @@ enum lf_field {
     LF_FIELD_KEYS = 0,        /* uint32[n]: cell key of each particle, previous order (NULL_HASH outside) */
     LF_FIELD_START = 1,       /* uint32[ncells+2]: cell start table; [ncells] = first out-of-grid slot */
     LF_FIELD_SORTED_IDS = 2,  /* uint32[n]: particle id in each slot of the new cell order */
-    LF_FIELD_DENSITY = 3,     /* float4[n]: (density, pressure, sound speed, 1/density^2), cell order */
+    LF_FIELD_DENSITY = 3,     /* float4[n]: (density, pressure, sound speed, 1/density), cell order */
     LF_FIELD_NORMAL = 4,      /* float4[n]: (normal xyz, 0), cell order */
     LF_FIELD_FORCES = 5,      /* float4[4n]: pressure, viscosity, tension, adhesion per particle
-                                 (written only with lf_set_debug(s,1)), cell order */
+                                 (written only with LF_DEBUG_FORCES), cell order */
     LF_FIELD_VEL_HALF = 6,    /* float4[n]: velocity after integration, before XSPH, cell order */
     LF_FIELD_SORTED_POS = 7   /* float4[n]: pre-integration positions in cell order */
 };
 /* Copies `bytes` bytes of a raw device field into dst (synchronous). */
 int lf_get_field(lf_scene *s, int field, void *dst, size_t bytes);
-/* 1 = also write the four force components (LF_FIELD_FORCES) each substep. */
-int lf_set_debug(lf_scene *s, int on);
+/* Debug flags: LF_DEBUG_FORCES also writes the four force components
+ * (LF_FIELD_FORCES) each substep; LF_DEBUG_NO_LISTS makes the normals, forces
+ * and XSPH passes scan the cell stencil instead of the neighbour list built
+ * by the density pass (results are bitwise the same either way). */
+#define LF_DEBUG_FORCES 1
+#define LF_DEBUG_NO_LISTS 2
+int lf_set_debug(lf_scene *s, int flags);
 /* Neighbour sets of the density pass for the current state (same FP32
  * predicate as the kernels), by particle id: counts[n], and when list is not
  * NULL the ids of each set, ascending, concatenated in id order.

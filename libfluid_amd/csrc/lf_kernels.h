@@ -8,30 +8,39 @@
 
 namespace lf {
 
+// Stream plus optional start/stop events bound to the first/last dispatch of
+// a launcher (hipExtLaunchKernelGGL).
+struct Launch {
+    hipStream_t st;
+    hipEvent_t a, b;
+};
+
 uint32_t scan_tiles(uint32_t m);
 
-void launch_hash(hipStream_t st, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
+void launch_hash(const Launch &L, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts);
-void launch_scan(hipStream_t st, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
-void launch_scatter(hipStream_t st, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
+void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
+void launch_scatter(const Launch &L, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id);
-void launch_gather(hipStream_t st, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
+                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
                    uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
-void launch_density(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                    float4 *D);
-void launch_normals(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
-                    uint32_t n, float4 *N);
-void launch_forces(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
-                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg);
-void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
-                 const float4 *D, const uint32_t *start, uint32_t n, float4 *Vout);
+void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
+                    uint32_t *nb, uint32_t *nbc, uint32_t *dmax);
+void launch_normals(const Launch &L, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
+                    const uint32_t *nb, const uint32_t *nbc, uint32_t n, float4 *N);
+void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
+                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, const uint32_t *nb,
+                   const uint32_t *nbc, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
+                   float4 *Fdbg, uint32_t *dmax);
+void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
+                 const float4 *D, const uint32_t *start, const uint32_t *nb, const uint32_t *nbc,
+                 const uint32_t *dmax, uint32_t n, float4 *Vout);
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A, float4 *L);
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *L, float4 *D);
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+                   float4 *A, float4 *Lw);
+void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *Lw, float4 *D);
+void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
                         const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2);
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
                    const float4 *D, uint32_t n, int by_id, float *out, uint32_t *ids);
@@ -39,6 +48,6 @@ void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const ui
                      uint32_t *cnt);
 void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
                     const uint64_t *off, uint32_t *list);
-void launch_copy(hipStream_t st, const float4 *src, float4 *dst, size_t n4);
+void launch_copy(const Launch &L, const float4 *src, float4 *dst, size_t n4);
 
 }  // namespace lf

@@ -8,8 +8,18 @@
 // first slot of every bucket (ncells grid cells + one out-of-grid bucket)
 // plus an end sentinel.
 //
+// The density pass tests every candidate of the 27-cell stencil and records
+// the slots within the skin radius h_s >= h in a k-major neighbour list
+// (nb[k * n + i], at most kMaxNb entries, in stencil order).  The normals,
+// forces and XSPH passes walk that list and apply their exact predicates, so
+// they visit the same neighbours in the same order as a stencil scan and give
+// bitwise the same sums; a particle whose list overflowed, or an XSPH pass
+// whose displacement bound is not met, falls back to the stencil scan.
+//
 // SPH here is a gather-bound stencil, not a dense contraction: the kernels
-// are VALU/L1 work over a 27-cell neighbour walk and never use MFMA.
+// are VALU/L1 work over a neighbour walk and never use MFMA.
+
+#include <hip/hip_ext.h>
 
 #include "lf_device.h"
 #include "lf_kernels.h"
@@ -178,27 +188,55 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84).
+// Neighbour walk shared by the passes after density: the particle's list when
+// `use_list` and it did not overflow, else the clipped 3x3x3 stencil of the
+// cell of `pcell` (minus i itself).  Both visit candidates in stencil order.
+template <class F>
+__device__ __forceinline__ void for_neighbours(const Consts &c, const uint32_t *__restrict__ start,
+                                               const uint32_t *__restrict__ nb, uint32_t cnt, uint32_t n,
+                                               uint32_t i, bool use_list, float4 pcell, F &&f) {
+    if (use_list && cnt <= (uint32_t)kMaxNb) {
+        for (uint32_t k = 0; k < cnt; ++k) f(nb[(size_t)k * n + i]);
+    } else {
+        int cx, cy, cz;
+        cell_xyz(c, pcell, cx, cy, cz);
+        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+            if (j != i) f(j);
+        });
+    }
+}
+
+// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84), and the
+// neighbour list of the substep.
 __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__restrict__ Ps,
                                                     const uint32_t *__restrict__ start, uint32_t n,
-                                                    float4 *__restrict__ D) {
+                                                    float4 *__restrict__ D, uint32_t *__restrict__ nb,
+                                                    uint32_t *__restrict__ nbc, uint32_t *__restrict__ dmax) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dmax = 0u;  // displacement bound of this substep, filled by k_forces
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     if (i >= start[c.ncells]) {  // out-of-grid bucket
         D[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        nbc[i] = 0u;
         return;
     }
     const float4 pi = Ps[i];
     int cx, cy, cz;
     cell_xyz(c, pi, cx, cy, cz);
     float S = 0.0f;
+    uint32_t cnt = 0;
     for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
         const float rr = dist2(pi, Ps[j]);
-        if (rr <= c.h2 && j != i) {
-            const float t = c.h2 - rr;
-            S += t * t * t;
+        if (rr <= c.h2s && j != i) {
+            if (cnt < (uint32_t)kMaxNb) nb[(size_t)cnt * n + i] = j;
+            ++cnt;
+            if (rr <= c.h2) {
+                const float t = c.h2 - rr;
+                S += t * t * t;
+            }
         }
     });
+    nbc[i] = cnt;
     S *= c.mC6;  // sum of m * poly6
     // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
     // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.
@@ -221,7 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
 // gradPoly6's factor r.
 __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Ps,
                                                     const float4 *__restrict__ D,
-                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    const uint32_t *__restrict__ start,
+                                                    const uint32_t *__restrict__ nb,
+                                                    const uint32_t *__restrict__ nbc, uint32_t n,
                                                     float4 *__restrict__ N) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -230,13 +270,11 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
         return;
     }
     const float4 pi = Ps[i];
-    int cx, cy, cz;
-    cell_xyz(c, pi, cx, cy, cz);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+    for_neighbours(c, start, nb, nbc[i], n, i, c.use_lists, pi, [&](uint32_t j) {
         const float4 q = Ps[j];
         const float rr = dist2(pi, q);
-        if (rr <= c.h2 && rr > c.eps && j != i) {
+        if (rr <= c.h2 && rr > c.eps) {
             const float4 dj = D[j];
             if (dj.x > c.eps) {
                 const float t = c.h2 - rr;
@@ -254,145 +292,160 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
+// Also records max |x_new - x_old| over the substep (as float bits) for the
+// XSPH pass's neighbour-list bound.
 __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
                                                    const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                    const float4 *__restrict__ Ls, const float4 *__restrict__ D,
                                                    const float4 *__restrict__ N, const uint32_t *__restrict__ start,
+                                                   const uint32_t *__restrict__ nb, const uint32_t *__restrict__ nbc,
                                                    uint32_t n, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
                                                    float4 *__restrict__ Aout, float4 *__restrict__ Lout,
-                                                   float4 *__restrict__ Fdbg) {
+                                                   float4 *__restrict__ Fdbg, uint32_t *__restrict__ dmax) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
-    float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
-    if (i < start[c.ncells]) {
-        const float4 di = D[i], ni = N[i];
-        const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
-        const float iri2 = iri * iri;
-        float Px = 0.f, Py = 0.f, Pz = 0.f;     // pressure sum
-        float Qx = 0.f, Qy = 0.f, Qz = 0.f;     // artificial-viscosity part of the viscosity sum
-        float Tx = 0.f, Ty = 0.f, Tz = 0.f;     // cohesion + curvature sum
-        int cx, cy, cz;
-        cell_xyz(c, pi, cx, cy, cz);
-        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
-            const float4 q = Ps[j];
-            const float rr = dist2(pi, q);
-            if (!(rr <= c.h2 && rr > c.eps && j != i)) return;
-            const float4 dj = D[j];
-            if (!(dj.x > c.eps)) return;
-            const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
-            const float inv_r = frsq(rr);
-            const float r = rr * inv_r;
-            const float hr = fmaxf(c.h - r, 0.0f);
-            const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
-            const float irj = dj.w;
-            const float mid = dj.y * iri2 + p_i * (irj * irj);
-            const float cp = cm * mid;
-            Px += cp * rx;
-            Py += cp * ry;
-            Pz += cp * rz;
-            if (c.has_visc) {
-                const float4 vj = Vs[j];
-                const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
-                if (ux < 0.0f) {
-                    const float mu = c.h * ux * frcp(rr + c.visc_eps);
-                    const float Pi = -c.nu * (cs_i + dj.z) * mu * frcp(rho_i + dj.x);
-                    const float cv = cm * Pi;
-                    Qx += cv * rx;
-                    Qy += cv * ry;
-                    Qz += cv * rz;
+    uint32_t disp_bits = 0u;
+    if (i < n) {
+        const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
+        float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
+        if (i < start[c.ncells]) {
+            const float4 di = D[i], ni = N[i];
+            const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
+            const float iri2 = iri * iri;
+            float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
+            float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
+            float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
+            for_neighbours(c, start, nb, nbc[i], n, i, c.use_lists, pi, [&](uint32_t j) {
+                const float4 q = Ps[j];
+                const float rr = dist2(pi, q);
+                if (!(rr <= c.h2 && rr > c.eps)) return;
+                const float4 dj = D[j];
+                if (!(dj.x > c.eps)) return;
+                const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
+                const float inv_r = frsq(rr);
+                const float r = rr * inv_r;
+                const float hr = fmaxf(c.h - r, 0.0f);
+                const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
+                const float irj = dj.w;
+                const float mid = dj.y * iri2 + p_i * (irj * irj);
+                const float cp = cm * mid;
+                Px += cp * rx;
+                Py += cp * ry;
+                Pz += cp * rz;
+                if (c.has_visc) {
+                    const float4 vj = Vs[j];
+                    const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
+                    if (ux < 0.0f) {
+                        const float mu = c.h * ux * frcp(rr + c.visc_eps);
+                        const float Pi = -c.nu * (cs_i + dj.z) * mu * frcp(rho_i + dj.x);
+                        const float cv = cm * Pi;
+                        Qx += cv * rx;
+                        Qy += cv * ry;
+                        Qz += cv * rz;
+                    }
+                }
+                if (c.has_tension) {
+                    const float4 nj = N[j];
+                    const float hr3r3 = hr * hr * hr * r * r * r;
+                    const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
+                    const float ch = -c.m2 * coh * inv_r;
+                    Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
+                    Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
+                    Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
+                }
+            });
+            fP = make_float3(Px, Py, Pz);
+            fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
+            // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
+            const float ts = c.sigma * c.rho0 * iri;
+            fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
+            if (c.has_adhesion) {
+                // Adhesion toward the walls (fluidphysics.cu:330-350), FP64.
+                const double z = -fmin(box_sdf(c, pi.x, pi.y, pi.z), 0.0) * c.quarter_inv_psize;
+                if (z < 1.0) {
+                    double nbv[3];
+                    box_normal(c, pi.x, pi.y, pi.z, nbv);
+                    const double oz = 1.0 - z;
+                    const double s = c.alpha_inv_rho0 * (double)rho_i / (oz * oz * oz);
+                    fA = make_float3((float)(s * nbv[0]), (float)(s * nbv[1]), (float)(s * nbv[2]));
                 }
             }
-            if (c.has_tension) {
-                const float4 nj = N[j];
-                const float hr3r3 = hr * hr * hr * r * r * r;
-                const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
-                const float ch = -c.m2 * coh * inv_r;
-                Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
-                Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
-                Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
-            }
-        });
-        fP = make_float3(Px, Py, Pz);
-        fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
-        // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
-        const float ts = c.sigma * c.rho0 * iri;
-        fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
-        if (c.has_adhesion) {
-            // Adhesion toward the walls (fluidphysics.cu:330-350), FP64.
-            const double z = -fmin(box_sdf(c, pi.x, pi.y, pi.z), 0.0) * c.quarter_inv_psize;
-            if (z < 1.0) {
-                double nb[3];
-                box_normal(c, pi.x, pi.y, pi.z, nb);
-                const double oz = 1.0 - z;
-                const double s = c.alpha_inv_rho0 * (double)rho_i / (oz * oz * oz);
-                fA = make_float3((float)(s * nb[0]), (float)(s * nb[1]), (float)(s * nb[2]));
-            }
         }
+        if (Fdbg) {
+            Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
+            Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
+            Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
+            Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
+        }
+        // Leapfrog (fluidintegrator.cu:36-69), per particle in FP64 on the FP32
+        // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
+        // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
+        // the boundary projection and reflection.
+        const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
+        const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
+        const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
+        double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
+        double s2 = ax * ax + ay * ay + az * az;
+        if (s2 > c.alim2) {
+            const double k = c.alim / sqrt(s2);
+            ax *= k;
+            ay *= k;
+            az *= k;
+        }
+        const float4 li = Ls[i];
+        double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
+        double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
+        double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
+        double vx = (double)vi.x + 0.5 * ((double)ai.x + ax) * c.dt;
+        double vy = (double)vi.y + 0.5 * ((double)ai.y + ay) * c.dt;
+        double vz = (double)vi.z + 0.5 * ((double)ai.z + az) * c.dt;
+        s2 = vx * vx + vy * vy + vz * vz;
+        if (s2 > c.vlim2) {
+            const double k = c.vlim / sqrt(s2);
+            vx *= k;
+            vy *= k;
+            vz *= k;
+        }
+        // boundaryCheckSDF (fluidintegrator.cu:11-26).
+        const double dd = box_sdf(c, x, y, z);
+        if (dd > -c.psize) {
+            double nbv[3];
+            box_normal(c, x, y, z, nbv);
+            const double push = dd + c.psize;
+            const double vn = c.vdamp1 * (nbv[0] * vx + nbv[1] * vy + nbv[2] * vz);
+            x -= push * nbv[0];
+            y -= push * nbv[1];
+            z -= push * nbv[2];
+            vx -= vn * nbv[0];
+            vy -= vn * nbv[1];
+            vz -= vn * nbv[2];
+        }
+        const float hx = (float)x, hy = (float)y, hz = (float)z;
+        Pout[i] = make_float4(hx, hy, hz, pi.w);
+        Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
+        Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
+        Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
+        // Displacement of the FP32 positions the pair predicates see, rounded up.
+        const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
+        const float disp = (float)(sqrt(ex * ex + ey * ey + ez * ez) * (1.0 + 1e-6));
+        disp_bits = __float_as_uint(disp);  // non-negative: bit order = value order (NaN sorts above all)
     }
-    if (Fdbg) {
-        Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
-        Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
-        Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
-        Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
-    }
-    // Leapfrog (fluidintegrator.cu:36-69), per particle in FP64 on the FP32
-    // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
-    // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
-    // the boundary projection and reflection.
-    const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
-    const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
-    const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
-    double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
-    double s2 = ax * ax + ay * ay + az * az;
-    if (s2 > c.alim2) {
-        const double k = c.alim / sqrt(s2);
-        ax *= k;
-        ay *= k;
-        az *= k;
-    }
-    const float4 li = Ls[i];
-    double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
-    double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
-    double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
-    double vx = (double)vi.x + 0.5 * ((double)ai.x + ax) * c.dt;
-    double vy = (double)vi.y + 0.5 * ((double)ai.y + ay) * c.dt;
-    double vz = (double)vi.z + 0.5 * ((double)ai.z + az) * c.dt;
-    s2 = vx * vx + vy * vy + vz * vz;
-    if (s2 > c.vlim2) {
-        const double k = c.vlim / sqrt(s2);
-        vx *= k;
-        vy *= k;
-        vz *= k;
-    }
-    // boundaryCheckSDF (fluidintegrator.cu:11-26).
-    const double dd = box_sdf(c, x, y, z);
-    if (dd > -c.psize) {
-        double nb[3];
-        box_normal(c, x, y, z, nb);
-        const double push = dd + c.psize;
-        const double vn = c.vdamp1 * (nb[0] * vx + nb[1] * vy + nb[2] * vz);
-        x -= push * nb[0];
-        y -= push * nb[1];
-        z -= push * nb[2];
-        vx -= vn * nb[0];
-        vy -= vn * nb[1];
-        vz -= vn * nb[2];
-    }
-    const float hx = (float)x, hy = (float)y, hz = (float)z;
-    Pout[i] = make_float4(hx, hy, hz, pi.w);
-    Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
-    Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
-    Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) disp_bits = max(disp_bits, (uint32_t)__shfl_xor(disp_bits, o, 64));
+    if ((threadIdx.x & 63) == 0 && disp_bits) atomicMax(dmax, disp_bits);
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
 // the stencil of the pre-integration cell, integrated positions, and the
-// pre-XSPH velocity of every neighbour (Jacobi update into Vout).
+// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  The list
+// of the substep holds every stencil slot within h_s of the old position; a
+// pair within h after the move is within h + 2 max|dx| before it, so the list
+// is complete when that is at most h_s (FP32 rounding margins included).
 __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restrict__ Ps,
                                                  const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
                                                  const float4 *__restrict__ D, const uint32_t *__restrict__ start,
-                                                 uint32_t n, float4 *__restrict__ Vout) {
+                                                 const uint32_t *__restrict__ nb, const uint32_t *__restrict__ nbc,
+                                                 const uint32_t *__restrict__ dmax, uint32_t n,
+                                                 float4 *__restrict__ Vout) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     const float4 vi = Vh[i];
@@ -400,14 +453,15 @@ __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restr
         Vout[i] = vi;
         return;
     }
+    const double dm = (double)__uint_as_float(*dmax);
+    const double reach = (double)c.h * (1.0 + 1e-6) + 2.0 * dm;
+    const bool list_ok = c.use_lists && (reach * reach * (1.0 + 1e-6) <= (double)c.h2s);
     const float4 xi = Pn[i];
     const float rho_i = D[i].x;
-    int cx, cy, cz;
-    cell_xyz(c, Ps[i], cx, cy, cz);
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+    for_neighbours(c, start, nb, nbc[i], n, i, list_ok, Ps[i], [&](uint32_t j) {
         const float rr = dist2(xi, Pn[j]);
-        if (rr <= c.h2 && j != i) {
+        if (rr <= c.h2) {
             const float sr = rho_i + D[j].x;
             if (sr > c.eps) {
                 const float t = c.h2 - rr;
@@ -528,76 +582,80 @@ __global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src,
 }
 
 // ---------------------------------------------------------------------------
-// Host launchers.
+// Host launchers.  Every launch goes through hipExtLaunchKernelGGL so that
+// optional start/stop events are bound to the dispatch packet itself.
 
 static inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
-void launch_hash(hipStream_t st, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
+#define LF_EXT(L, kern, nblocks, ev0, ev1, ...) \
+    hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, (L).st, (ev0), (ev1), 0, __VA_ARGS__)
+
+void launch_hash(const Launch &L, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts) {
-    if (n) hipLaunchKernelGGL(k_hash, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, P, n, keys, ranks, counts);
+    if (n) LF_EXT(L, k_hash, blocks_for(n), L.a, L.b, c, P, n, keys, ranks, counts);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
-void launch_scan(hipStream_t st, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
+void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
     const uint32_t nb = scan_tiles(m);
-    hipLaunchKernelGGL(k_scan_sum, dim3(nb), dim3(kBlock), 0, st, counts, m, bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, st, bsum, nb, start, m);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, st, counts, m, bsum, start);
+    LF_EXT(L, k_scan_sum, nb, L.a, nullptr, (const uint32_t *)counts, m, bsum);
+    LF_EXT(L, k_scan_top, 1, nullptr, nullptr, bsum, nb, start, m);
+    LF_EXT(L, k_scan_down, nb, nullptr, L.b, counts, m, (const uint32_t *)bsum, start);
 }
 
-void launch_scatter(hipStream_t st, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
+void launch_scatter(const Launch &L, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id) {
-    if (n)
-        hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, keys, ranks, start, n, slot_src,
-                           slot_id);
+    if (n) LF_EXT(L, k_scatter, blocks_for(n), L.a, L.b, P, keys, ranks, start, n, slot_src, slot_id);
 }
 
-void launch_gather(hipStream_t st, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
+                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
                    uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err) {
     if (n)
-        hipLaunchKernelGGL(k_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, slot_src, slot_id, keys, start, P, V,
-                           A, L, n, Ps, Vs, As, Ls, err);
+        LF_EXT(L, k_gather, blocks_for(n), L.a, L.b, slot_src, slot_id, keys, start, P, V, A, Lw, n, Ps, Vs, As, Ls,
+               err);
 }
 
-void launch_density(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                    float4 *D) {
-    if (n) hipLaunchKernelGGL(k_density, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, D);
+void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
+                    uint32_t *nb, uint32_t *nbc, uint32_t *dmax) {
+    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, nb, nbc, dmax);
 }
 
-void launch_normals(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
-                    uint32_t n, float4 *N) {
-    if (n) hipLaunchKernelGGL(k_normals, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, D, start, n, N);
+void launch_normals(const Launch &L, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
+                    const uint32_t *nb, const uint32_t *nbc, uint32_t n, float4 *N) {
+    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Ps, D, start, nb, nbc, n, N);
 }
 
-void launch_forces(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
-                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg) {
+void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
+                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, const uint32_t *nb,
+                   const uint32_t *nbc, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
+                   float4 *Fdbg, uint32_t *dmax) {
     if (n)
-        hipLaunchKernelGGL(k_forces, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Vs, As, Ls, D, N, start, n,
-                           Pout, Vout, Aout, Lout, Fdbg);
+        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Vs, As, Ls, D, N, start, nb, nbc, n, Pout, Vout, Aout,
+               Lout, Fdbg, dmax);
 }
 
-void launch_xsph(hipStream_t st, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
-                 const float4 *D, const uint32_t *start, uint32_t n, float4 *Vout) {
-    if (n) hipLaunchKernelGGL(k_xsph, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, Pn, Vh, D, start, n, Vout);
+void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
+                 const float4 *D, const uint32_t *start, const uint32_t *nb, const uint32_t *nbc,
+                 const uint32_t *dmax, uint32_t n, float4 *Vout) {
+    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, Pn, Vh, D, start, nb, nbc, dmax, n, Vout);
 }
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A, float4 *L) {
+                   float4 *A, float4 *Lw) {
     if (count)
-        hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A, L);
+        hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A, Lw);
 }
 
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *L, float4 *D) {
-    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, L, D);
+void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *Lw, float4 *D) {
+    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, Lw, D);
 }
 
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *L,
+void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
                         const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2) {
     if (n)
-        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, L, D, n, P2, V2, A2, L2,
+        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, Lw, D, n, P2, V2, A2, L2,
                            D2);
 }
 
@@ -616,10 +674,10 @@ void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uin
     if (n) hipLaunchKernelGGL(k_nb_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, off, list);
 }
 
-void launch_copy(hipStream_t st, const float4 *src, float4 *dst, size_t n4) {
+void launch_copy(const Launch &L, const float4 *src, float4 *dst, size_t n4) {
     const size_t want = (n4 + kBlock - 1) / kBlock;
     const unsigned nb = (unsigned)(want < 65536 ? (want ? want : 1) : 65536);
-    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, st, src, dst, n4);
+    LF_EXT(L, k_copy, nb, L.a, L.b, src, dst, n4);
 }
 
 }  // namespace lf

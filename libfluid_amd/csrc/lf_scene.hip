@@ -54,6 +54,7 @@ struct lf_scene {
     float4 *D = nullptr, *N = nullptr, *F = nullptr;
     uint32_t *keys = nullptr, *ranks = nullptr, *slot_src = nullptr, *slot_id = nullptr;
     uint32_t *counts = nullptr, *start = nullptr, *bsum = nullptr, *err = nullptr;
+    uint32_t *nb = nullptr, *nbc = nullptr, *dmax = nullptr;  // neighbour lists of the substep
     // profiling
     uint32_t prof_mask = 0;
     std::vector<ProfRec> prof_live;
@@ -76,7 +77,8 @@ int dalloc(T **p, size_t count) {
 
 void free_all(lf_scene *s) {
     void *bufs[] = {s->P0, s->P1, s->V0, s->V1, s->V2, s->A0, s->A1, s->L0, s->L1, s->D, s->N, s->F,
-                    s->keys, s->ranks, s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err};
+                    s->keys, s->ranks, s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err,
+                    s->nb, s->nbc, s->dmax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &r : s->prof_live) {
@@ -90,7 +92,7 @@ void free_all(lf_scene *s) {
 }
 
 // Kernel constants for one substep of length dt (host double -> float once).
-lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt) {
+lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool use_lists = true) {
     lf::Consts c;
     std::memset(&c, 0, sizeof(c));
     for (int a = 0; a < 3; ++a) {
@@ -111,6 +113,13 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt) {
     c.h2 = p.h * p.h;  // FP32 product, the pair-predicate threshold (m_h2)
     c.inv_h = (float)(1.0 / h);
     c.eps = FLT_MIN;
+    // Neighbour-list radius h + skin.  The skin covers twice the largest
+    // expected move of a substep (velocity and acceleration at their limits)
+    // with margin, and at least 0.1 h; XSPH checks the actual move.
+    const double move = (double)p.vel_limit * dt + 0.5 * (double)p.acc_limit * (double)dt * dt;
+    const double skin = std::min(h, std::max(0.1 * h, 2.5 * move));
+    c.h2s = (float)((h + skin) * (h + skin));
+    c.use_lists = use_lists ? 1 : 0;
     c.mC6 = (float)(m * C6);
     c.rho0 = p.rest_density;
     const double inv_rho0 = p.rest_density == 0.0f ? 1.0 : 1.0 / (double)p.rest_density;
@@ -161,50 +170,55 @@ hipEvent_t take_event(lf_scene *s) {
     return e;
 }
 
-// Runs `launch`, bracketed by events when kernel class `kind` is profiled.
-template <class L>
-void timed(lf_scene *s, int kind, L &&launch) {
+// Runs `launch(L)`; when kernel class `kind` is profiled, L carries start and
+// stop events that the launcher binds to its first and last dispatch.
+template <class Fn>
+void timed(lf_scene *s, int kind, Fn &&launch) {
+    lf::Launch L{s->stream, nullptr, nullptr};
     if (s->prof_mask & (1u << kind)) {
         hipEvent_t a = take_event(s), b = take_event(s);
         if (a && b) {
-            (void)hipEventRecord(a, s->stream);
-            launch();
-            (void)hipEventRecord(b, s->stream);
+            L.a = a;
+            L.b = b;
+            launch(L);
             s->prof_live.push_back(ProfRec{kind, a, b});
             return;
         }
         if (a) s->ev_pool.push_back(a);
         if (b) s->ev_pool.push_back(b);
     }
-    launch();
+    launch(L);
 }
 
 // Hash, scan, provisional scatter and stable gather of the current state
 // (P0,V0,A0) into cell order (P1,V1,A1).
 void enqueue_sort(lf_scene *s, const lf::Consts &c) {
-    hipStream_t st = s->stream;
-    timed(s, LF_K_HASH, [&] { lf::launch_hash(st, c, s->P0, s->n, s->keys, s->ranks, s->counts); });
-    timed(s, LF_K_SCAN, [&] { lf::launch_scan(st, s->counts, s->nbuckets, s->bsum, s->start); });
-    timed(s, LF_K_SCATTER,
-          [&] { lf::launch_scatter(st, s->P0, s->keys, s->ranks, s->start, s->n, s->slot_src, s->slot_id); });
-    timed(s, LF_K_GATHER, [&] {
-        lf::launch_gather(st, s->slot_src, s->slot_id, s->keys, s->start, s->P0, s->V0, s->A0, s->L0, s->n, s->P1,
+    timed(s, LF_K_HASH, [&](const lf::Launch &L) { lf::launch_hash(L, c, s->P0, s->n, s->keys, s->ranks, s->counts); });
+    timed(s, LF_K_SCAN, [&](const lf::Launch &L) { lf::launch_scan(L, s->counts, s->nbuckets, s->bsum, s->start); });
+    timed(s, LF_K_SCATTER, [&](const lf::Launch &L) {
+        lf::launch_scatter(L, s->P0, s->keys, s->ranks, s->start, s->n, s->slot_src, s->slot_id);
+    });
+    timed(s, LF_K_GATHER, [&](const lf::Launch &L) {
+        lf::launch_gather(L, s->slot_src, s->slot_id, s->keys, s->start, s->P0, s->V0, s->A0, s->L0, s->n, s->P1,
                           s->V1, s->A1, s->L1, s->err);
     });
 }
 
-// One substep.  Buffer roles: state (P0,V0,A0) -> sorted (P1,V1,A1) ->
-// integrated P0, A0 and pre-XSPH velocity V2 -> XSPH velocity V0.
+// One substep.  Buffer roles: state (P0,V0,A0,L0) -> sorted (P1,V1,A1,L1) ->
+// integrated P0, A0, L0 and pre-XSPH velocity V2 -> XSPH velocity V0.
 void enqueue_substep(lf_scene *s, const lf::Consts &c) {
-    hipStream_t st = s->stream;
     enqueue_sort(s, c);
-    timed(s, LF_K_DENSITY, [&] { lf::launch_density(st, c, s->P1, s->start, s->n, s->D); });
-    timed(s, LF_K_NORMALS, [&] { lf::launch_normals(st, c, s->P1, s->D, s->start, s->n, s->N); });
-    timed(s, LF_K_FORCES, [&] {
-        lf::launch_forces(st, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->n, s->P0, s->V2, s->A0,
-                          s->L0, s->debug ? s->F : nullptr);
+    timed(s, LF_K_DENSITY,
+          [&](const lf::Launch &L) { lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->nb, s->nbc, s->dmax); });
+    timed(s, LF_K_NORMALS,
+          [&](const lf::Launch &L) { lf::launch_normals(L, c, s->P1, s->D, s->start, s->nb, s->nbc, s->n, s->N); });
+    timed(s, LF_K_FORCES, [&](const lf::Launch &L) {
+        lf::launch_forces(L, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->nb, s->nbc, s->n, s->P0, s->V2,
+                          s->A0, s->L0, (s->debug & 1) ? s->F : nullptr, s->dmax);
     });
-    timed(s, LF_K_XSPH, [&] { lf::launch_xsph(st, c, s->P1, s->P0, s->V2, s->D, s->start, s->n, s->V0); });
+    timed(s, LF_K_XSPH, [&](const lf::Launch &L) {
+        lf::launch_xsph(L, c, s->P1, s->P0, s->V2, s->D, s->start, s->nb, s->nbc, s->dmax, s->n, s->V0);
+    });
     s->identity = false;
 }
 
@@ -276,6 +290,9 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     if (!rc) rc = dalloc(&s->start, (size_t)s->nbuckets + 1);
     if (!rc) rc = dalloc(&s->bsum, (size_t)s->ntiles + 1);
     if (!rc) rc = dalloc(&s->err, 1);
+    if (!rc) rc = dalloc(&s->nb, (size_t)lf::kMaxNb * n);
+    if (!rc) rc = dalloc(&s->nbc, n);
+    if (!rc) rc = dalloc(&s->dmax, 1);
     if (!rc) {
         // Counts start at zero; afterwards every scan clears them.
         e = hipMemsetAsync(s->counts, 0, sizeof(uint32_t) * s->nbuckets, s->stream);
@@ -399,7 +416,7 @@ int lf_step(lf_scene *s, float dt, uint32_t substeps) {
     // substeps run (the reference's float-accumulated loop, :301, can run one
     // fewer).
     const float part = dt / float(substeps);
-    const lf::Consts c = make_consts(s->params, s->ncells, part);
+    const lf::Consts c = make_consts(s->params, s->ncells, part, (s->debug & 2) == 0);
     for (uint32_t k = 0; k < substeps; ++k) {
         enqueue_substep(s, c);
         ++s->steps;
@@ -426,11 +443,12 @@ int lf_step_count(const lf_scene *s, uint64_t *count) {
     return LF_OK;
 }
 
-int lf_set_debug(lf_scene *s, int on) {
+int lf_set_debug(lf_scene *s, int flags) {
     if (!s) return fail(LF_ERR_INVALID, "null scene");
-    if (on && !s->F)
+    if (flags & ~3) return fail(LF_ERR_INVALID, "lf_set_debug: unknown flags %d", flags);
+    if ((flags & 1) && !s->F)
         if (int rc = dalloc(&s->F, 4 * (size_t)s->n)) return rc;
-    s->debug = on ? 1 : 0;
+    s->debug = flags;
     return LF_OK;
 }
 
@@ -575,9 +593,10 @@ int lf_copy_probe(lf_scene *s, size_t bytes, int repeats, double *ms) {
     if (e == hipSuccess) e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) {
-        lf::launch_copy(s->stream, a, b, bytes / 16);  // warm-up
+        const lf::Launch plain{s->stream, nullptr, nullptr};
+        lf::launch_copy(plain, a, b, bytes / 16);  // warm-up
         (void)hipEventRecord(e0, s->stream);
-        for (int r = 0; r < repeats; ++r) lf::launch_copy(s->stream, a, b, bytes / 16);
+        for (int r = 0; r < repeats; ++r) lf::launch_copy(plain, a, b, bytes / 16);
         (void)hipEventRecord(e1, s->stream);
         e = hipStreamSynchronize(s->stream);
     }
