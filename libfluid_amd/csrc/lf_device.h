@@ -10,7 +10,6 @@ namespace lf {
 constexpr int kBlock = 256;             // threads per workgroup for every particle kernel
 constexpr uint32_t kRankFixMax = 4096;  // largest bucket the in-cell id ranking handles
 constexpr int kScanPerThread = 16;      // cell-count scan: entries per thread
-constexpr int kMaxNb = 80;              // neighbour-list capacity per particle (overflow -> stencil scan)
 constexpr int kScanTile = kBlock * kScanPerThread;
 
 // Kernel-argument block: every constant a substep needs, derived on the host
@@ -22,9 +21,12 @@ struct Consts {
     int gx, gy, gz;
     uint32_t ncells;
     double half_box[3];
-    // smoothing length; h2s = list (skin) radius squared, >= h2
-    float h, h2, inv_h, eps, h2s;
-    int use_lists;
+    // smoothing length
+    float h, h2, inv_h, eps;
+    // stencil-cell skipping: cell sizes, FP32 margin on the face gaps, the
+    // squared reach of the static passes (>= h2 with rounding margin), on/off
+    float cell[3], cull_margin, reach2;
+    int cull;
     // density + EOS
     float mC6;        // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;

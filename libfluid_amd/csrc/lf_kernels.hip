@@ -8,16 +8,15 @@
 // first slot of every bucket (ncells grid cells + one out-of-grid bucket)
 // plus an end sentinel.
 //
-// The density pass tests every candidate of the 27-cell stencil and records
-// the slots within the skin radius h_s >= h in a k-major neighbour list
-// (nb[k * n + i], at most kMaxNb entries, in stencil order).  The normals,
-// forces and XSPH passes walk that list and apply their exact predicates, so
-// they visit the same neighbours in the same order as a stencil scan and give
-// bitwise the same sums; a particle whose list overflowed, or an XSPH pass
-// whose displacement bound is not met, falls back to the stencil scan.
+// The neighbour passes walk the clipped 27-cell stencil as nine contiguous
+// slot ranges with four position loads in flight, and skip the cells (and so
+// whole rows) whose nearest face is provably farther than the pass's reach
+// (h, or h plus twice the substep's largest move for XSPH).  Skipped
+// candidates are exactly ones the pair predicate would reject, so every pass
+// sums the same neighbours in the same order with or without the skipping.
 //
 // SPH here is a gather-bound stencil, not a dense contraction: the kernels
-// are VALU/L1 work over a neighbour walk and never use MFMA.
+// are VALU/L1 work over the stencil walk and never use MFMA.
 
 #include <hip/hip_ext.h>
 
@@ -188,55 +187,71 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour walk shared by the passes after density: the particle's list when
-// `use_list` and it did not overflow, else the clipped 3x3x3 stencil of the
-// cell of `pcell` (minus i itself).  Both visit candidates in stencil order.
+// Stencil walk shared by the four neighbour passes: calls f(j, Pos[j]) for
+// every slot j of the clipped 3x3x3 cell stencil around the cell of `pc`
+// (the position the sort used), in row order, skipping cells whose nearest
+// point lies farther than sqrt(R2) from pc.  Face gaps are shrunk by
+// c.cull_margin, which covers FP32 rounding of the cell boundaries
+// (floor(x * inv_cell)) and of the gaps, so a skipped candidate could not
+// pass the pair predicate.
 template <class F>
-__device__ __forceinline__ void for_neighbours(const Consts &c, const uint32_t *__restrict__ start,
-                                               const uint32_t *__restrict__ nb, uint32_t cnt, uint32_t n,
-                                               uint32_t i, bool use_list, float4 pcell, F &&f) {
-    if (use_list && cnt <= (uint32_t)kMaxNb) {
-        for (uint32_t k = 0; k < cnt; ++k) f(nb[(size_t)k * n + i]);
-    } else {
-        int cx, cy, cz;
-        cell_xyz(c, pcell, cx, cy, cz);
-        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
-            if (j != i) f(j);
-        });
+__device__ __forceinline__ void walk_stencil(const Consts &c, const uint32_t *__restrict__ start,
+                                             const float4 *__restrict__ Pos, float4 pc, float R2, F &&f) {
+    int cx, cy, cz;
+    cell_xyz(c, pc, cx, cy, cz);
+    const float m = c.cull_margin;
+    const float lx = fmaxf(pc.x - (float)cx * c.cell[0] - m, 0.0f), hx = fmaxf((float)(cx + 1) * c.cell[0] - pc.x - m, 0.0f);
+    const float ly = fmaxf(pc.y - (float)cy * c.cell[1] - m, 0.0f), hy = fmaxf((float)(cy + 1) * c.cell[1] - pc.y - m, 0.0f);
+    const float lz = fmaxf(pc.z - (float)cz * c.cell[2] - m, 0.0f), hz = fmaxf((float)(cz + 1) * c.cell[2] - pc.z - m, 0.0f);
+    const float lx2 = c.cull ? lx * lx : 0.0f, hx2 = c.cull ? hx * hx : 0.0f;
+    const float ly2 = c.cull ? ly * ly : 0.0f, hy2 = c.cull ? hy * hy : 0.0f;
+    const float lz2 = c.cull ? lz * lz : 0.0f, hz2 = c.cull ? hz * hz : 0.0f;
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= c.gz) continue;
+        const float gz2 = dz < 0 ? lz2 : (dz > 0 ? hz2 : 0.0f);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = cy + dy;
+            if (y < 0 || y >= c.gy) continue;
+            const float gyz2 = gz2 + (dy < 0 ? ly2 : (dy > 0 ? hy2 : 0.0f));
+            if (gyz2 > R2) continue;
+            const int xl = (cx > 0 && gyz2 + lx2 <= R2) ? cx - 1 : cx;
+            const int xh = (cx < c.gx - 1 && gyz2 + hx2 <= R2) ? cx + 1 : cx;
+            const uint32_t row = (uint32_t)(z * c.gy + y) * (uint32_t)c.gx;
+            uint32_t j = start[row + (uint32_t)xl];
+            const uint32_t e = start[row + (uint32_t)xh + 1u];
+            for (; j + 4u <= e; j += 4u) {
+                const float4 q0 = Pos[j], q1 = Pos[j + 1], q2 = Pos[j + 2], q3 = Pos[j + 3];
+                f(j, q0);
+                f(j + 1u, q1);
+                f(j + 2u, q2);
+                f(j + 3u, q3);
+            }
+            for (; j < e; ++j) f(j, Pos[j]);
+        }
     }
 }
 
-// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84), and the
-// neighbour list of the substep.
+// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84).
 __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__restrict__ Ps,
                                                     const uint32_t *__restrict__ start, uint32_t n,
-                                                    float4 *__restrict__ D, uint32_t *__restrict__ nb,
-                                                    uint32_t *__restrict__ nbc, uint32_t *__restrict__ dmax) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *dmax = 0u;  // displacement bound of this substep, filled by k_forces
+                                                    float4 *__restrict__ D, uint32_t *__restrict__ dmax) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dmax = 0u;  // largest move of this substep, filled by k_forces
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     if (i >= start[c.ncells]) {  // out-of-grid bucket
         D[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        nbc[i] = 0u;
         return;
     }
     const float4 pi = Ps[i];
-    int cx, cy, cz;
-    cell_xyz(c, pi, cx, cy, cz);
     float S = 0.0f;
-    uint32_t cnt = 0;
-    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
-        const float rr = dist2(pi, Ps[j]);
-        if (rr <= c.h2s && j != i) {
-            if (cnt < (uint32_t)kMaxNb) nb[(size_t)cnt * n + i] = j;
-            ++cnt;
-            if (rr <= c.h2) {
-                const float t = c.h2 - rr;
-                S += t * t * t;
-            }
+    walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
+        const float rr = dist2(pi, q);
+        if (rr <= c.h2 && j != i) {
+            const float t = c.h2 - rr;
+            S += t * t * t;
         }
     });
-    nbc[i] = cnt;
     S *= c.mC6;  // sum of m * poly6
     // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
     // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.
@@ -259,9 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
 // gradPoly6's factor r.
 __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Ps,
                                                     const float4 *__restrict__ D,
-                                                    const uint32_t *__restrict__ start,
-                                                    const uint32_t *__restrict__ nb,
-                                                    const uint32_t *__restrict__ nbc, uint32_t n,
+                                                    const uint32_t *__restrict__ start, uint32_t n,
                                                     float4 *__restrict__ N) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -271,10 +284,9 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
     }
     const float4 pi = Ps[i];
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    for_neighbours(c, start, nb, nbc[i], n, i, c.use_lists, pi, [&](uint32_t j) {
-        const float4 q = Ps[j];
+    walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
         const float rr = dist2(pi, q);
-        if (rr <= c.h2 && rr > c.eps) {
+        if (rr <= c.h2 && rr > c.eps && j != i) {
             const float4 dj = D[j];
             if (dj.x > c.eps) {
                 const float t = c.h2 - rr;
@@ -292,13 +304,12 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
-// Also records max |x_new - x_old| over the substep (as float bits) for the
-// XSPH pass's neighbour-list bound.
+// Also records max |x_new - x_old| over the substep (as float bits), which
+// bounds the XSPH pass's cell skipping.
 __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
                                                    const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                    const float4 *__restrict__ Ls, const float4 *__restrict__ D,
                                                    const float4 *__restrict__ N, const uint32_t *__restrict__ start,
-                                                   const uint32_t *__restrict__ nb, const uint32_t *__restrict__ nbc,
                                                    uint32_t n, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
                                                    float4 *__restrict__ Aout, float4 *__restrict__ Lout,
                                                    float4 *__restrict__ Fdbg, uint32_t *__restrict__ dmax) {
@@ -314,10 +325,9 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
             float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
             float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
             float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
-            for_neighbours(c, start, nb, nbc[i], n, i, c.use_lists, pi, [&](uint32_t j) {
-                const float4 q = Ps[j];
+            walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
                 const float rr = dist2(pi, q);
-                if (!(rr <= c.h2 && rr > c.eps)) return;
+                if (!(rr <= c.h2 && rr > c.eps && j != i)) return;
                 const float4 dj = D[j];
                 if (!(dj.x > c.eps)) return;
                 const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
@@ -424,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
         Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
         Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
         Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
-        // Displacement of the FP32 positions the pair predicates see, rounded up.
+        // Move of the FP32 positions the pair predicates see, rounded up.
         const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
         const float disp = (float)(sqrt(ex * ex + ey * ey + ez * ez) * (1.0 + 1e-6));
         disp_bits = __float_as_uint(disp);  // non-negative: bit order = value order (NaN sorts above all)
@@ -436,14 +446,12 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
 // the stencil of the pre-integration cell, integrated positions, and the
-// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  The list
-// of the substep holds every stencil slot within h_s of the old position; a
-// pair within h after the move is within h + 2 max|dx| before it, so the list
-// is complete when that is at most h_s (FP32 rounding margins included).
+// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  A pair
+// within h after the move is within h + 2 max|dx| before it, which is the
+// reach used to skip cells (measured on the old positions the cells hold).
 __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restrict__ Ps,
                                                  const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
                                                  const float4 *__restrict__ D, const uint32_t *__restrict__ start,
-                                                 const uint32_t *__restrict__ nb, const uint32_t *__restrict__ nbc,
                                                  const uint32_t *__restrict__ dmax, uint32_t n,
                                                  float4 *__restrict__ Vout) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
@@ -453,15 +461,16 @@ __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restr
         Vout[i] = vi;
         return;
     }
-    const double dm = (double)__uint_as_float(*dmax);
-    const double reach = (double)c.h * (1.0 + 1e-6) + 2.0 * dm;
-    const bool list_ok = c.use_lists && (reach * reach * (1.0 + 1e-6) <= (double)c.h2s);
+    const double reach = sqrt((double)c.reach2) + 2.0 * (double)__uint_as_float(*dmax);
+    const float R2 = (float)(reach * reach * (1.0 + 1e-6));  // NaN or inf: no cell is skipped
     const float4 xi = Pn[i];
     const float rho_i = D[i].x;
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    for_neighbours(c, start, nb, nbc[i], n, i, list_ok, Ps[i], [&](uint32_t j) {
-        const float rr = dist2(xi, Pn[j]);
-        if (rr <= c.h2) {
+    // The walk loads old positions (Ps) only to bound cells; the predicate
+    // uses the integrated ones.
+    walk_stencil(c, start, Pn, Ps[i], isfinite(R2) ? R2 : 3.0e38f, [&](uint32_t j, float4 q) {
+        const float rr = dist2(xi, q);
+        if (rr <= c.h2 && j != i) {
             const float sr = rho_i + D[j].x;
             if (sr > c.eps) {
                 const float t = c.h2 - rr;
@@ -618,28 +627,26 @@ void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *sl
 }
 
 void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
-                    uint32_t *nb, uint32_t *nbc, uint32_t *dmax) {
-    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, nb, nbc, dmax);
+                    uint32_t *dmax) {
+    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, dmax);
 }
 
 void launch_normals(const Launch &L, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
-                    const uint32_t *nb, const uint32_t *nbc, uint32_t n, float4 *N) {
-    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Ps, D, start, nb, nbc, n, N);
+                    uint32_t n, float4 *N) {
+    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Ps, D, start, n, N);
 }
 
 void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, const uint32_t *nb,
-                   const uint32_t *nbc, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
-                   float4 *Fdbg, uint32_t *dmax) {
+                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
+                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg, uint32_t *dmax) {
     if (n)
-        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Vs, As, Ls, D, N, start, nb, nbc, n, Pout, Vout, Aout,
-               Lout, Fdbg, dmax);
+        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Vs, As, Ls, D, N, start, n, Pout, Vout, Aout, Lout, Fdbg,
+               dmax);
 }
 
 void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
-                 const float4 *D, const uint32_t *start, const uint32_t *nb, const uint32_t *nbc,
-                 const uint32_t *dmax, uint32_t n, float4 *Vout) {
-    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, Pn, Vh, D, start, nb, nbc, dmax, n, Vout);
+                 const float4 *D, const uint32_t *start, const uint32_t *dmax, uint32_t n, float4 *Vout) {
+    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, Pn, Vh, D, start, dmax, n, Vout);
 }
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,

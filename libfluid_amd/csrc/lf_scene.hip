@@ -54,7 +54,7 @@ struct lf_scene {
     float4 *D = nullptr, *N = nullptr, *F = nullptr;
     uint32_t *keys = nullptr, *ranks = nullptr, *slot_src = nullptr, *slot_id = nullptr;
     uint32_t *counts = nullptr, *start = nullptr, *bsum = nullptr, *err = nullptr;
-    uint32_t *nb = nullptr, *nbc = nullptr, *dmax = nullptr;  // neighbour lists of the substep
+    uint32_t *dmax = nullptr;  // largest particle move of the substep (float bits)
     // profiling
     uint32_t prof_mask = 0;
     std::vector<ProfRec> prof_live;
@@ -78,7 +78,7 @@ int dalloc(T **p, size_t count) {
 void free_all(lf_scene *s) {
     void *bufs[] = {s->P0, s->P1, s->V0, s->V1, s->V2, s->A0, s->A1, s->L0, s->L1, s->D, s->N, s->F,
                     s->keys, s->ranks, s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err,
-                    s->nb, s->nbc, s->dmax};
+                    s->dmax};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &r : s->prof_live) {
@@ -92,7 +92,7 @@ void free_all(lf_scene *s) {
 }
 
 // Kernel constants for one substep of length dt (host double -> float once).
-lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool use_lists = true) {
+lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool cull = true) {
     lf::Consts c;
     std::memset(&c, 0, sizeof(c));
     for (int a = 0; a < 3; ++a) {
@@ -113,13 +113,19 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool use_l
     c.h2 = p.h * p.h;  // FP32 product, the pair-predicate threshold (m_h2)
     c.inv_h = (float)(1.0 / h);
     c.eps = FLT_MIN;
-    // Neighbour-list radius h + skin.  The skin covers twice the largest
-    // expected move of a substep (velocity and acceleration at their limits)
-    // with margin, and at least 0.1 h; XSPH checks the actual move.
-    const double move = (double)p.vel_limit * dt + 0.5 * (double)p.acc_limit * (double)dt * dt;
-    const double skin = std::min(h, std::max(0.1 * h, 2.5 * move));
-    c.h2s = (float)((h + skin) * (h + skin));
-    c.use_lists = use_lists ? 1 : 0;
+    // Stencil-cell skipping.  A cell is skipped when its nearest face, less
+    // the margin, is farther than sqrt(reach2) from the particle; the margin
+    // covers FP32 rounding of floor(x * inv_cell) cell boundaries and of the
+    // gap arithmetic (a few ulps of the largest coordinate), and reach2
+    // exceeds h^2 by more than the FP32 pair predicate's rounding.
+    double box_max = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        c.cell[a] = (float)(1.0 / (double)p.inv_cell[a]);
+        box_max = std::max(box_max, std::max((double)p.box[a], (double)p.grid[a] / (double)p.inv_cell[a]));
+    }
+    c.cull_margin = (float)(1e-6 * box_max);
+    c.reach2 = (float)((double)c.h2 * (1.0 + 1e-6));
+    c.cull = cull ? 1 : 0;
     c.mC6 = (float)(m * C6);
     c.rho0 = p.rest_density;
     const double inv_rho0 = p.rest_density == 0.0f ? 1.0 : 1.0 / (double)p.rest_density;
@@ -208,17 +214,14 @@ void enqueue_sort(lf_scene *s, const lf::Consts &c) {
 // integrated P0, A0, L0 and pre-XSPH velocity V2 -> XSPH velocity V0.
 void enqueue_substep(lf_scene *s, const lf::Consts &c) {
     enqueue_sort(s, c);
-    timed(s, LF_K_DENSITY,
-          [&](const lf::Launch &L) { lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->nb, s->nbc, s->dmax); });
-    timed(s, LF_K_NORMALS,
-          [&](const lf::Launch &L) { lf::launch_normals(L, c, s->P1, s->D, s->start, s->nb, s->nbc, s->n, s->N); });
+    timed(s, LF_K_DENSITY, [&](const lf::Launch &L) { lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->dmax); });
+    timed(s, LF_K_NORMALS, [&](const lf::Launch &L) { lf::launch_normals(L, c, s->P1, s->D, s->start, s->n, s->N); });
     timed(s, LF_K_FORCES, [&](const lf::Launch &L) {
-        lf::launch_forces(L, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->nb, s->nbc, s->n, s->P0, s->V2,
-                          s->A0, s->L0, (s->debug & 1) ? s->F : nullptr, s->dmax);
+        lf::launch_forces(L, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->n, s->P0, s->V2, s->A0, s->L0,
+                          (s->debug & 1) ? s->F : nullptr, s->dmax);
     });
-    timed(s, LF_K_XSPH, [&](const lf::Launch &L) {
-        lf::launch_xsph(L, c, s->P1, s->P0, s->V2, s->D, s->start, s->nb, s->nbc, s->dmax, s->n, s->V0);
-    });
+    timed(s, LF_K_XSPH,
+          [&](const lf::Launch &L) { lf::launch_xsph(L, c, s->P1, s->P0, s->V2, s->D, s->start, s->dmax, s->n, s->V0); });
     s->identity = false;
 }
 
@@ -290,8 +293,6 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     if (!rc) rc = dalloc(&s->start, (size_t)s->nbuckets + 1);
     if (!rc) rc = dalloc(&s->bsum, (size_t)s->ntiles + 1);
     if (!rc) rc = dalloc(&s->err, 1);
-    if (!rc) rc = dalloc(&s->nb, (size_t)lf::kMaxNb * n);
-    if (!rc) rc = dalloc(&s->nbc, n);
     if (!rc) rc = dalloc(&s->dmax, 1);
     if (!rc) {
         // Counts start at zero; afterwards every scan clears them.

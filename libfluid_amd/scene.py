@@ -105,9 +105,9 @@ class Scene:
         return c.value
 
     # -- inspection ---------------------------------------------------------
-    def set_debug(self, forces: bool = True, no_lists: bool = False) -> None:
-        """forces: keep the four force components; no_lists: stencil scans only."""
-        check(lib.lf_set_debug(self._h, (1 if forces else 0) | (2 if no_lists else 0)), "lf_set_debug")
+    def set_debug(self, forces: bool = True, no_cull: bool = False) -> None:
+        """forces: keep the four force components; no_cull: walk all 27 stencil cells."""
+        check(lib.lf_set_debug(self._h, (1 if forces else 0) | (2 if no_cull else 0)), "lf_set_debug")
 
     def field(self, which: int) -> np.ndarray:
         n, nc = self.n, self.params.ncells

@@ -234,20 +234,20 @@ def test_full_size_column_c2(gpu, oracle_omp):
 
 
 @pytest.mark.parametrize("case", ["c1", "double_dam"])
-def test_neighbour_lists_are_bitwise_transparent(gpu, case):
-    """The list walk of the normals/forces/XSPH passes gives bitwise the same
-    trajectory as scanning the cell stencil (same neighbours, same order)."""
+def test_cell_skipping_is_bitwise_transparent(gpu, case):
+    """Skipping out-of-reach stencil cells gives bitwise the same trajectory as
+    visiting all 27 (same neighbours, same order)."""
     cfg = configs.dambreak_c1() if case == "c1" else configs.double_dam(n=4096, res=16)
     outs = []
-    for no_lists in (False, True):
+    for no_cull in (False, True):
         sc = gpu.Scene(cfg.params, cfg.n)
         try:
             sc.upload(cfg.pos, cfg.vel)
-            sc.set_debug(forces=False, no_lists=no_lists)
+            sc.set_debug(forces=False, no_cull=no_cull)
             sc.run(cfg.dt, 20)
             sc.sync()
             outs.append(sc.download(L.LF_ORDER_ID))
         finally:
             sc.close()
     for k in ("pos", "vel", "acc", "colour"):
-        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs between list and stencil walks"
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs with and without cell skipping"
