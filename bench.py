@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample duration")
-    ap.add_argument("--no-cull", action="store_true", help="walk all 27 stencil cells (no cell skipping)")
+    ap.add_argument("--scan", action="store_true",
+                    help="test every stencil slot in every pass instead of walking neighbour masks")
     return ap.parse_args()
 
 
@@ -107,8 +108,8 @@ def main():
     log(f"[bench] {cfg.name}: {n} particles, grid {list(cfg.params.grid)}, warmup {args.warmup}, steps {args.steps}")
     sc = lf.Scene(cfg.params, n)
     sc.upload(cfg.pos, cfg.vel)
-    if args.no_cull:
-        sc.set_debug(forces=False, no_cull=True)
+    if args.scan:
+        sc.set_debug(forces=False, scan=True)
 
     for _ in range(args.warmup):
         sc.step(cfg.dt, 1)

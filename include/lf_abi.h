@@ -133,11 +133,12 @@ enum lf_field {
 /* Copies `bytes` bytes of a raw device field into dst (synchronous). */
 int lf_get_field(lf_scene *s, int field, void *dst, size_t bytes);
 /* Debug flags: LF_DEBUG_FORCES also writes the four force components
- * (LF_FIELD_FORCES) each substep; LF_DEBUG_NO_CULL makes the neighbour passes
- * visit every cell of the 27-cell stencil instead of skipping cells that are
- * provably out of reach (results are bitwise the same either way). */
+ * (LF_FIELD_FORCES) each substep; LF_DEBUG_SCAN makes the normals, forces and
+ * XSPH passes test every slot of the 27-cell stencil instead of walking the
+ * neighbour masks the density pass records (results are bitwise the same
+ * either way). */
 #define LF_DEBUG_FORCES 1
-#define LF_DEBUG_NO_CULL 2
+#define LF_DEBUG_SCAN 2
 int lf_set_debug(lf_scene *s, int flags);
 /* Neighbour sets of the density pass for the current state (same FP32
  * predicate as the kernels), by particle id: counts[n], and when list is not

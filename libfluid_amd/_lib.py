@@ -34,6 +34,9 @@ FIELD_FORCES = 5
 FIELD_VEL_HALF = 6
 FIELD_SORTED_POS = 7
 
+DEBUG_FORCES = 1
+DEBUG_SCAN = 2
+
 KERNELS = ("hash", "scan", "scatter", "gather", "density", "normals", "forces", "xsph")
 K_INDEX = {k: i for i, k in enumerate(KERNELS)}
 

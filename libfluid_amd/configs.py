@@ -113,4 +113,16 @@ def scaled_c1(nx: int, ny: int, nz: int, seed: int = 7) -> Config:
     return Config(f"block_{nx}x{ny}x{nz}", p, pos, np.zeros_like(pos), 1e-4, 10)
 
 
+def dense_block(nx: int = 30, ny: int = 30, nz: int = 30, spacing: float = 0.25 * H, seed: int = 11) -> Config:
+    """A finer lattice (default spacing h/4, about 64 particles a cell) at rest
+    density (m = rho0 spacing^3, particle size spacing/2), offset from the
+    cell grid so that stencil rows inside hold more than 64 slots and rows at
+    its faces fewer; in the C1 box."""
+    p = _plain_params((1.0, 0.5, 0.5), (50, 25, 25))
+    p.mass = RHO0 * spacing ** 3
+    p.particle_size = spacing / 2
+    pos = jittered_lattice(nx, ny, nz, spacing, (0.503, 0.107, 0.111), 0.05 * spacing, seed)
+    return Config(f"dense_{nx}x{ny}x{nz}", p, pos, np.zeros_like(pos), 1e-4, 10)
+
+
 CONFIGS = {"c1": dambreak_c1, "c2": column_c2, "c3": spray_c3, "double_dam": double_dam}

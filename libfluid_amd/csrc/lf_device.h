@@ -15,6 +15,15 @@ constexpr int kScanTile = kBlock * kScanPerThread;
 // Kernel-argument block: every constant a substep needs, derived on the host
 // (in double, rounded once to float) from lf_params.  Passed by value, so a
 // launch reads it through the kernarg segment (scalar loads).
+// Neighbour masks of one radius for 9 stencil rows x n particles, row-major
+// (entry r * n + i).  Bit k stands for slot b_r + k of row r.  A row of at
+// most 32 slots keeps its mask in w32, a row of 33..64 slots in w64; longer
+// rows have no mask.
+struct RowMasks {
+    uint32_t *w32;
+    uint64_t *w64;
+};
+
 struct Consts {
     // grid (FluidParamData m_res/m_res2/m_numCells, generalised per axis)
     float inv_cell[3];
@@ -23,10 +32,13 @@ struct Consts {
     double half_box[3];
     // smoothing length
     float h, h2, inv_h, eps;
-    // stencil-cell skipping: cell sizes, FP32 margin on the face gaps, the
-    // squared reach of the static passes (>= h2 with rounding margin), on/off
-    float cell[3], cull_margin, reach2;
-    int cull;
+    // neighbour masks: squared skin radius (>= h2) of the XSPH masks; the
+    // largest particle move (FP32 positions, FP64 distance) for which the
+    // skin masks still hold every XSPH neighbour; whether the normals, forces
+    // and XSPH passes walk masks at all (off: every row is scanned)
+    float h2s;
+    double move_limit;
+    int use_masks;
     // density + EOS
     float mC6;        // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;

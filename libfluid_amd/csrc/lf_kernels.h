@@ -26,14 +26,15 @@ void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *sl
                    const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
                    uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
 void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
-                    uint32_t *dmax);
-void launch_normals(const Launch &L, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
+                    float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move);
+void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uint32_t *start, RowMasks M,
                     uint32_t n, float4 *N);
-void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
-                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg, uint32_t *dmax);
-void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
-                 const float4 *D, const uint32_t *start, const uint32_t *dmax, uint32_t n, float4 *Vout);
+void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Q, const float *Pr,
+                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, const float4 *N,
+                   const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
+                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move);
+void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,
+                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, uint32_t n, float4 *Vout);
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
                    float4 *A, float4 *Lw);

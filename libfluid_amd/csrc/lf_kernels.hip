@@ -8,12 +8,13 @@
 // first slot of every bucket (ncells grid cells + one out-of-grid bucket)
 // plus an end sentinel.
 //
-// The neighbour passes walk the clipped 27-cell stencil as nine contiguous
-// slot ranges with four position loads in flight, and skip the cells (and so
-// whole rows) whose nearest face is provably farther than the pass's reach
-// (h, or h plus twice the substep's largest move for XSPH).  Skipped
-// candidates are exactly ones the pair predicate would reject, so every pass
-// sums the same neighbours in the same order with or without the skipping.
+// The density pass tests every candidate of the clipped 27-cell stencil (nine
+// contiguous x-row slot ranges) and records, per row, 64-bit masks of the
+// slots within h and within a skin radius.  The normals and forces passes walk
+// the exact masks, XSPH the skin masks (when the substep's largest move
+// proves them complete), over packed neighbour data; set bits are visited in
+// slot order, so every pass sums the same neighbours in the same order as a
+// full stencil scan, bit for bit.
 //
 // SPH here is a gather-bound stencil, not a dense contraction: the kernels
 // are VALU/L1 work over the stencil walk and never use MFMA.
@@ -187,71 +188,106 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Stencil walk shared by the four neighbour passes: calls f(j, Pos[j]) for
-// every slot j of the clipped 3x3x3 cell stencil around the cell of `pc`
-// (the position the sort used), in row order, skipping cells whose nearest
-// point lies farther than sqrt(R2) from pc.  Face gaps are shrunk by
-// c.cull_margin, which covers FP32 rounding of the cell boundaries
-// (floor(x * inv_cell)) and of the gaps, so a skipped candidate could not
-// pass the pair predicate.
-template <class F>
-__device__ __forceinline__ void walk_stencil(const Consts &c, const uint32_t *__restrict__ start,
-                                             const float4 *__restrict__ Pos, float4 pc, float R2, F &&f) {
+// Neighbour rows.  The clipped 27-cell stencil of a cell is nine contiguous
+// slot ranges (x-rows, z outer, y inner: the reference's visiting order of
+// cells, fluidphysics.cu:30-53, up to the order inside a row).  rows9 issues
+// the eighteen start-table loads together; rows outside the grid are empty.
+__device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restrict__ start, float4 pc, uint32_t *b,
+                                      uint32_t *e) {
     int cx, cy, cz;
     cell_xyz(c, pc, cx, cy, cz);
-    const float m = c.cull_margin;
-    const float lx = fmaxf(pc.x - (float)cx * c.cell[0] - m, 0.0f), hx = fmaxf((float)(cx + 1) * c.cell[0] - pc.x - m, 0.0f);
-    const float ly = fmaxf(pc.y - (float)cy * c.cell[1] - m, 0.0f), hy = fmaxf((float)(cy + 1) * c.cell[1] - pc.y - m, 0.0f);
-    const float lz = fmaxf(pc.z - (float)cz * c.cell[2] - m, 0.0f), hz = fmaxf((float)(cz + 1) * c.cell[2] - pc.z - m, 0.0f);
-    const float lx2 = c.cull ? lx * lx : 0.0f, hx2 = c.cull ? hx * hx : 0.0f;
-    const float ly2 = c.cull ? ly * ly : 0.0f, hy2 = c.cull ? hy * hy : 0.0f;
-    const float lz2 = c.cull ? lz * lz : 0.0f, hz2 = c.cull ? hz * hz : 0.0f;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int z = cz + dz;
-        if (z < 0 || z >= c.gz) continue;
-        const float gz2 = dz < 0 ? lz2 : (dz > 0 ? hz2 : 0.0f);
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int y = cy + dy;
-            if (y < 0 || y >= c.gy) continue;
-            const float gyz2 = gz2 + (dy < 0 ? ly2 : (dy > 0 ? hy2 : 0.0f));
-            if (gyz2 > R2) continue;
-            const int xl = (cx > 0 && gyz2 + lx2 <= R2) ? cx - 1 : cx;
-            const int xh = (cx < c.gx - 1 && gyz2 + hx2 <= R2) ? cx + 1 : cx;
-            const uint32_t row = (uint32_t)(z * c.gy + y) * (uint32_t)c.gx;
-            uint32_t j = start[row + (uint32_t)xl];
-            const uint32_t e = start[row + (uint32_t)xh + 1u];
-            for (; j + 4u <= e; j += 4u) {
-                const float4 q0 = Pos[j], q1 = Pos[j + 1], q2 = Pos[j + 2], q3 = Pos[j + 3];
-                f(j, q0);
-                f(j + 1u, q1);
-                f(j + 2u, q2);
-                f(j + 3u, q3);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < c.gx - 1 ? cx + 1 : c.gx - 1;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
+        const bool ok = z >= 0 && z < c.gz && y >= 0 && y < c.gy;
+        const uint32_t row = (uint32_t)((ok ? z : cz) * c.gy + (ok ? y : cy)) * (uint32_t)c.gx;
+        const uint32_t bb = start[row + (uint32_t)x0];
+        const uint32_t ee = start[row + (uint32_t)x1 + 1u];
+        b[r] = bb;
+        e[r] = ok ? ee : bb;
+    }
+}
+
+// The density pass records, per row, a 64-bit mask of the row's slots that
+// are neighbours (bit k = slot b[r] + k), once for radius h and once for the
+// skin radius.  walk_masks visits the set bits in increasing slot order, the
+// same order as a full scan; a row longer than 64 slots has no mask and is
+// scanned in full.  With use == false every row is scanned.
+template <class F>
+__device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
+                                           uint32_t n, uint32_t i, F &&f) {
+    uint64_t mk[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t len = e[r] - b[r];
+        const size_t at = (size_t)r * n + i;
+        mk[r] = 0ull;
+        if (use && len <= 32u) mk[r] = mask.w32[at];
+        else if (use && len <= 64u) mk[r] = mask.w64[at];
+    }
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        if (use && e[r] - b[r] <= 64u) {
+            uint64_t m = mk[r];
+            while (m) {
+                const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                m &= m - 1ull;
+                f(b[r] + k);
             }
-            for (; j < e; ++j) f(j, Pos[j]);
+        } else {
+            for (uint32_t j = b[r]; j < e[r]; ++j)
+                if (j != i) f(j);
         }
     }
 }
 
-// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84).
+// 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
+// neighbour masks of the substep and the packed neighbour data of the later
+// passes: Q = (x, y, z, 1/density, or -1 when density <= FLT_MIN), Pr =
+// pressure.
 __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__restrict__ Ps,
                                                     const uint32_t *__restrict__ start, uint32_t n,
-                                                    float4 *__restrict__ D, uint32_t *__restrict__ dmax) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *dmax = 0u;  // largest move of this substep, filled by k_forces
+                                                    float4 *__restrict__ D, float4 *__restrict__ Q,
+                                                    float *__restrict__ Pr, RowMasks M, RowMasks MS,
+                                                    uint32_t *__restrict__ far_move) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *far_move = 0u;  // set by k_forces when the skin is exceeded
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
-    if (i >= start[c.ncells]) {  // out-of-grid bucket
+    if (i >= start[c.ncells]) {  // out-of-grid bucket: never a neighbour, no masks
         D[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
     const float4 pi = Ps[i];
+    uint32_t b[9], e[9];
+    rows9(c, start, pi, b, e);
     float S = 0.0f;
-    walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
-        const float rr = dist2(pi, q);
-        if (rr <= c.h2 && j != i) {
-            const float t = c.h2 - rr;
-            S += t * t * t;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint64_t m = 0ull, ms = 0ull;
+        for (uint32_t j = b[r]; j < e[r]; ++j) {
+            const float rr = dist2(pi, Ps[j]);
+            const uint32_t k = j - b[r];
+            const uint64_t bit = k < 64u ? (1ull << k) : 0ull;
+            if (j != i) {
+                if (rr <= c.h2) {
+                    const float t = c.h2 - rr;
+                    S += t * t * t;
+                    m |= bit;
+                }
+                if (rr <= c.h2s) ms |= bit;
+            }
         }
-    });
+        const uint32_t len = e[r] - b[r];
+        const size_t at = (size_t)r * n + i;
+        if (len <= 32u) {
+            M.w32[at] = (uint32_t)m;
+            MS.w32[at] = (uint32_t)ms;
+        } else if (len <= 64u) {
+            M.w64[at] = m;
+            MS.w64[at] = ms;
+        }
+    }
     S *= c.mC6;  // sum of m * poly6
     // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
     // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.
@@ -266,36 +302,38 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
     const float ratio = corrected * c.inv_rho0;
     const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (powf(ratio, c.gamma) - 1.0f);
     const float cs = c.K * powf(ratio, c.gamma_m1);
-    D[i] = make_float4(rho, press, cs, 1.0f / rho);
+    const float ir = 1.0f / rho;
+    D[i] = make_float4(rho, press, cs, ir);
+    Q[i] = make_float4(pi.x, pi.y, pi.z, rho > c.eps ? ir : -1.0f);
+    Pr[i] = press;
 }
 
 // 6. surface normals (computeNormals, fluidphysics.cu:154-210):
 // n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
-// gradPoly6's factor r.
-__global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Ps,
-                                                    const float4 *__restrict__ D,
-                                                    const uint32_t *__restrict__ start, uint32_t n,
-                                                    float4 *__restrict__ N) {
+// gradPoly6's factor r.  Neighbours with density <= FLT_MIN (Q.w < 0) or at
+// r^2 <= FLT_MIN contribute zero.
+__global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Q,
+                                                    const uint32_t *__restrict__ start, RowMasks M,
+                                                    uint32_t n, float4 *__restrict__ N) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
     if (i >= start[c.ncells]) {
         N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
-    const float4 pi = Ps[i];
+    const float4 pi = Q[i];
+    uint32_t b[9], e[9];
+    rows9(c, start, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
+    walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) {
+        const float4 q = Q[j];
         const float rr = dist2(pi, q);
-        if (rr <= c.h2 && rr > c.eps && j != i) {
-            const float4 dj = D[j];
-            if (dj.x > c.eps) {
-                const float t = c.h2 - rr;
-                const float w = t * t * dj.w;
-                sx += w * (pi.x - q.x);
-                sy += w * (pi.y - q.y);
-                sz += w * (pi.z - q.z);
-            }
-        }
+        float t = fmaxf(c.h2 - rr, 0.0f);
+        t = rr > c.eps ? t : 0.0f;
+        const float w = t * t * fmaxf(q.w, 0.0f);
+        sx += w * (pi.x - q.x);
+        sy += w * (pi.y - q.y);
+        sz += w * (pi.z - q.z);
     });
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
 }
@@ -304,45 +342,52 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
-// Also records max |x_new - x_old| over the substep (as float bits), which
-// bounds the XSPH pass's cell skipping.
+// Writes X = (integrated position, density) for the XSPH pass and sets
+// far_move when a particle moved farther than the XSPH skin masks allow.
 __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
+                                                   const float4 *__restrict__ Q, const float *__restrict__ Pr,
                                                    const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                    const float4 *__restrict__ Ls, const float4 *__restrict__ D,
                                                    const float4 *__restrict__ N, const uint32_t *__restrict__ start,
-                                                   uint32_t n, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
+                                                   RowMasks M, uint32_t n,
+                                                   float4 *__restrict__ Pout, float4 *__restrict__ Vout,
                                                    float4 *__restrict__ Aout, float4 *__restrict__ Lout,
-                                                   float4 *__restrict__ Fdbg, uint32_t *__restrict__ dmax) {
+                                                   float4 *__restrict__ X, float4 *__restrict__ Fdbg,
+                                                   uint32_t *__restrict__ far_move) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    uint32_t disp_bits = 0u;
     if (i < n) {
         const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
+        const float4 di = D[i];
         float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
         if (i < start[c.ncells]) {
-            const float4 di = D[i], ni = N[i];
+            const float4 ni = N[i];
             const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
             const float iri2 = iri * iri;
             float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
             float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
             float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
-            walk_stencil(c, start, Ps, pi, c.reach2, [&](uint32_t j, float4 q) {
+            uint32_t b[9], e[9];
+            rows9(c, start, pi, b, e);
+            walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) {
+                const float4 q = Q[j];
+                const float pj = Pr[j];
                 const float rr = dist2(pi, q);
-                if (!(rr <= c.h2 && rr > c.eps && j != i)) return;
-                const float4 dj = D[j];
-                if (!(dj.x > c.eps)) return;
+                if (!(rr <= c.h2 && rr > c.eps)) return;
+                if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
                 const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
                 const float inv_r = frsq(rr);
                 const float r = rr * inv_r;
                 const float hr = fmaxf(c.h - r, 0.0f);
                 const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
-                const float irj = dj.w;
-                const float mid = dj.y * iri2 + p_i * (irj * irj);
+                const float irj = q.w;
+                const float mid = pj * iri2 + p_i * (irj * irj);
                 const float cp = cm * mid;
                 Px += cp * rx;
                 Py += cp * ry;
                 Pz += cp * rz;
                 if (c.has_visc) {
                     const float4 vj = Vs[j];
+                    const float4 dj = D[j];
                     const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
                     if (ux < 0.0f) {
                         const float mu = c.h * ux * frcp(rr + c.visc_eps);
@@ -434,25 +479,27 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
         Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
         Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
         Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
-        // Move of the FP32 positions the pair predicates see, rounded up.
+        X[i] = make_float4(hx, hy, hz, di.x);
+        // Move of the FP32 positions the pair predicates see (the differences
+        // are exact in FP64).  Every writer stores the same value; NaN counts
+        // as too far.
         const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
-        const float disp = (float)(sqrt(ex * ex + ey * ey + ez * ez) * (1.0 + 1e-6));
-        disp_bits = __float_as_uint(disp);  // non-negative: bit order = value order (NaN sorts above all)
+        if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) *far_move = 1u;
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) disp_bits = max(disp_bits, (uint32_t)__shfl_xor(disp_bits, o, 64));
-    if ((threadIdx.x & 63) == 0 && disp_bits) atomicMax(dmax, disp_bits);
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
 // the stencil of the pre-integration cell, integrated positions, and the
-// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  A pair
-// within h after the move is within h + 2 max|dx| before it, which is the
-// reach used to skip cells (measured on the old positions the cells hold).
+// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  The skin
+// masks hold every stencil slot within the skin radius h_s before the move;
+// a pair within h after the move was within h + 2 max|dx| before it, so the
+// masks are complete while no particle moved farther than move_limit
+// (h + 2 move_limit <= h_s with FP32 margins); k_forces flags a farther move
+// in far_move, and then every row is scanned.
 __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restrict__ Ps,
-                                                 const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
-                                                 const float4 *__restrict__ D, const uint32_t *__restrict__ start,
-                                                 const uint32_t *__restrict__ dmax, uint32_t n,
+                                                 const float4 *__restrict__ X, const float4 *__restrict__ Vh,
+                                                 const uint32_t *__restrict__ start, RowMasks MS,
+                                                 const uint32_t *__restrict__ far_move, uint32_t n,
                                                  float4 *__restrict__ Vout) {
     const uint32_t i = xcd_block() * kBlock + threadIdx.x;
     if (i >= n) return;
@@ -461,17 +508,17 @@ __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restr
         Vout[i] = vi;
         return;
     }
-    const double reach = sqrt((double)c.reach2) + 2.0 * (double)__uint_as_float(*dmax);
-    const float R2 = (float)(reach * reach * (1.0 + 1e-6));  // NaN or inf: no cell is skipped
-    const float4 xi = Pn[i];
-    const float rho_i = D[i].x;
+    const bool masks_ok = c.use_masks && *far_move == 0u;
+    const float4 xi = X[i];
+    const float rho_i = xi.w;
+    uint32_t b[9], e[9];
+    rows9(c, start, Ps[i], b, e);
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    // The walk loads old positions (Ps) only to bound cells; the predicate
-    // uses the integrated ones.
-    walk_stencil(c, start, Pn, Ps[i], isfinite(R2) ? R2 : 3.0e38f, [&](uint32_t j, float4 q) {
+    walk_masks(b, e, MS, masks_ok, n, i, [&](uint32_t j) {
+        const float4 q = X[j];
         const float rr = dist2(xi, q);
-        if (rr <= c.h2 && j != i) {
-            const float sr = rho_i + D[j].x;
+        if (rr <= c.h2) {
+            const float sr = rho_i + q.w;
             if (sr > c.eps) {
                 const float t = c.h2 - rr;
                 const float w = t * t * t * frcp(sr);
@@ -627,26 +674,27 @@ void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *sl
 }
 
 void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
-                    uint32_t *dmax) {
-    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, dmax);
+                    float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move) {
+    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, Q, Pr, M, MS, far_move);
 }
 
-void launch_normals(const Launch &L, const Consts &c, const float4 *Ps, const float4 *D, const uint32_t *start,
+void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uint32_t *start, RowMasks M,
                     uint32_t n, float4 *N) {
-    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Ps, D, start, n, N);
+    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Q, start, M, n, N);
 }
 
-void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Vs, const float4 *As,
-                   const float4 *Ls, const float4 *D, const float4 *N, const uint32_t *start, uint32_t n,
-                   float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout, float4 *Fdbg, uint32_t *dmax) {
+void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Q, const float *Pr,
+                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, const float4 *N,
+                   const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
+                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move) {
     if (n)
-        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Vs, As, Ls, D, N, start, n, Pout, Vout, Aout, Lout, Fdbg,
-               dmax);
+        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout, Aout,
+               Lout, X, Fdbg, far_move);
 }
 
-void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Pn, const float4 *Vh,
-                 const float4 *D, const uint32_t *start, const uint32_t *dmax, uint32_t n, float4 *Vout) {
-    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, Pn, Vh, D, start, dmax, n, Vout);
+void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,
+                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, uint32_t n, float4 *Vout) {
+    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, X, Vh, start, MS, far_move, n, Vout);
 }
 
 void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,

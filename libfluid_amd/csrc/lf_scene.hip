@@ -32,6 +32,7 @@ using lf::fail;
 namespace {
 
 constexpr double kPi = 3.141592654;  // CUDART_PI_F as defined by the reference (fluiddefines.h:16)
+constexpr double kSkin = 0.1;
 
 struct ProfRec {
     int kind;
@@ -52,9 +53,15 @@ struct lf_scene {
     float4 *P0 = nullptr, *P1 = nullptr, *V0 = nullptr, *V1 = nullptr, *V2 = nullptr, *A0 = nullptr, *A1 = nullptr;
     float4 *L0 = nullptr, *L1 = nullptr;  // low-order position words
     float4 *D = nullptr, *N = nullptr, *F = nullptr;
+    // packed neighbour data: Q = (x, y, z, 1/density or -1), Pr = pressure,
+    // X = (integrated position, density); row masks within h (M) and within
+    // the skin radius (MS), 9 rows x n, row-major
+    float4 *Q = nullptr, *X = nullptr;
+    float *Pr = nullptr;
+    lf::RowMasks M{nullptr, nullptr}, MS{nullptr, nullptr};
     uint32_t *keys = nullptr, *ranks = nullptr, *slot_src = nullptr, *slot_id = nullptr;
     uint32_t *counts = nullptr, *start = nullptr, *bsum = nullptr, *err = nullptr;
-    uint32_t *dmax = nullptr;  // largest particle move of the substep (float bits)
+    uint32_t *far_move = nullptr;  // set when a particle moved farther than the XSPH skin allows
     // profiling
     uint32_t prof_mask = 0;
     std::vector<ProfRec> prof_live;
@@ -77,8 +84,8 @@ int dalloc(T **p, size_t count) {
 
 void free_all(lf_scene *s) {
     void *bufs[] = {s->P0, s->P1, s->V0, s->V1, s->V2, s->A0, s->A1, s->L0, s->L1, s->D, s->N, s->F,
-                    s->keys, s->ranks, s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err,
-                    s->dmax};
+                    s->Q, s->X, s->Pr, s->M.w32, s->M.w64, s->MS.w32, s->MS.w64, s->keys, s->ranks,
+                    s->slot_src, s->slot_id, s->counts, s->start, s->bsum, s->err, s->far_move};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &r : s->prof_live) {
@@ -92,7 +99,7 @@ void free_all(lf_scene *s) {
 }
 
 // Kernel constants for one substep of length dt (host double -> float once).
-lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool cull = true) {
+lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool use_masks = true) {
     lf::Consts c;
     std::memset(&c, 0, sizeof(c));
     for (int a = 0; a < 3; ++a) {
@@ -113,19 +120,14 @@ lf::Consts make_consts(const lf_params &p, uint32_t ncells, float dt, bool cull 
     c.h2 = p.h * p.h;  // FP32 product, the pair-predicate threshold (m_h2)
     c.inv_h = (float)(1.0 / h);
     c.eps = FLT_MIN;
-    // Stencil-cell skipping.  A cell is skipped when its nearest face, less
-    // the margin, is farther than sqrt(reach2) from the particle; the margin
-    // covers FP32 rounding of floor(x * inv_cell) cell boundaries and of the
-    // gap arithmetic (a few ulps of the largest coordinate), and reach2
-    // exceeds h^2 by more than the FP32 pair predicate's rounding.
-    double box_max = 1.0;
-    for (int a = 0; a < 3; ++a) {
-        c.cell[a] = (float)(1.0 / (double)p.inv_cell[a]);
-        box_max = std::max(box_max, std::max((double)p.box[a], (double)p.grid[a] / (double)p.inv_cell[a]));
-    }
-    c.cull_margin = (float)(1e-6 * box_max);
-    c.reach2 = (float)((double)c.h2 * (1.0 + 1e-6));
-    c.cull = cull ? 1 : 0;
+    // Skin radius of the XSPH masks: (1 + kSkin) h.  XSPH uses them while
+    // the largest move of a substep stays below about kSkin h / 2.
+    c.h2s = (float)((1.0 + kSkin) * (1.0 + kSkin) * (double)c.h2);
+    // A pair within h after the move (FP32 predicate: true distance below
+    // h (1 + 1e-6)) was within h (1 + 1e-6) + 2 d before it, which the FP32
+    // skin predicate accepts when that squared, times (1 + 1e-6), is <= h2s.
+    c.move_limit = 0.5 * (std::sqrt((double)c.h2s / (1.0 + 1e-6)) - (double)c.h * (1.0 + 1e-6)) * (1.0 - 1e-9);
+    c.use_masks = use_masks ? 1 : 0;
     c.mC6 = (float)(m * C6);
     c.rho0 = p.rest_density;
     const double inv_rho0 = p.rest_density == 0.0f ? 1.0 : 1.0 / (double)p.rest_density;
@@ -214,14 +216,17 @@ void enqueue_sort(lf_scene *s, const lf::Consts &c) {
 // integrated P0, A0, L0 and pre-XSPH velocity V2 -> XSPH velocity V0.
 void enqueue_substep(lf_scene *s, const lf::Consts &c) {
     enqueue_sort(s, c);
-    timed(s, LF_K_DENSITY, [&](const lf::Launch &L) { lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->dmax); });
-    timed(s, LF_K_NORMALS, [&](const lf::Launch &L) { lf::launch_normals(L, c, s->P1, s->D, s->start, s->n, s->N); });
-    timed(s, LF_K_FORCES, [&](const lf::Launch &L) {
-        lf::launch_forces(L, c, s->P1, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->n, s->P0, s->V2, s->A0, s->L0,
-                          (s->debug & 1) ? s->F : nullptr, s->dmax);
+    timed(s, LF_K_DENSITY, [&](const lf::Launch &L) {
+        lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->Q, s->Pr, s->M, s->MS, s->far_move);
     });
-    timed(s, LF_K_XSPH,
-          [&](const lf::Launch &L) { lf::launch_xsph(L, c, s->P1, s->P0, s->V2, s->D, s->start, s->dmax, s->n, s->V0); });
+    timed(s, LF_K_NORMALS, [&](const lf::Launch &L) { lf::launch_normals(L, c, s->Q, s->start, s->M, s->n, s->N); });
+    timed(s, LF_K_FORCES, [&](const lf::Launch &L) {
+        lf::launch_forces(L, c, s->P1, s->Q, s->Pr, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->M, s->n, s->P0,
+                          s->V2, s->A0, s->L0, s->X, (s->debug & LF_DEBUG_FORCES) ? s->F : nullptr, s->far_move);
+    });
+    timed(s, LF_K_XSPH, [&](const lf::Launch &L) {
+        lf::launch_xsph(L, c, s->P1, s->X, s->V2, s->start, s->MS, s->far_move, s->n, s->V0);
+    });
     s->identity = false;
 }
 
@@ -285,6 +290,13 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     if (!rc) rc = dalloc(&s->L1, n);
     if (!rc) rc = dalloc(&s->D, n);
     if (!rc) rc = dalloc(&s->N, n);
+    if (!rc) rc = dalloc(&s->Q, n);
+    if (!rc) rc = dalloc(&s->X, n);
+    if (!rc) rc = dalloc(&s->Pr, n);
+    if (!rc) rc = dalloc(&s->M.w32, 9 * n);
+    if (!rc) rc = dalloc(&s->M.w64, 9 * n);
+    if (!rc) rc = dalloc(&s->MS.w32, 9 * n);
+    if (!rc) rc = dalloc(&s->MS.w64, 9 * n);
     if (!rc) rc = dalloc(&s->keys, n);
     if (!rc) rc = dalloc(&s->ranks, n);
     if (!rc) rc = dalloc(&s->slot_src, n);
@@ -293,7 +305,7 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     if (!rc) rc = dalloc(&s->start, (size_t)s->nbuckets + 1);
     if (!rc) rc = dalloc(&s->bsum, (size_t)s->ntiles + 1);
     if (!rc) rc = dalloc(&s->err, 1);
-    if (!rc) rc = dalloc(&s->dmax, 1);
+    if (!rc) rc = dalloc(&s->far_move, 1);
     if (!rc) {
         // Counts start at zero; afterwards every scan clears them.
         e = hipMemsetAsync(s->counts, 0, sizeof(uint32_t) * s->nbuckets, s->stream);
@@ -417,7 +429,7 @@ int lf_step(lf_scene *s, float dt, uint32_t substeps) {
     // substeps run (the reference's float-accumulated loop, :301, can run one
     // fewer).
     const float part = dt / float(substeps);
-    const lf::Consts c = make_consts(s->params, s->ncells, part, (s->debug & 2) == 0);
+    const lf::Consts c = make_consts(s->params, s->ncells, part, (s->debug & LF_DEBUG_SCAN) == 0);
     for (uint32_t k = 0; k < substeps; ++k) {
         enqueue_substep(s, c);
         ++s->steps;
@@ -446,8 +458,8 @@ int lf_step_count(const lf_scene *s, uint64_t *count) {
 
 int lf_set_debug(lf_scene *s, int flags) {
     if (!s) return fail(LF_ERR_INVALID, "null scene");
-    if (flags & ~3) return fail(LF_ERR_INVALID, "lf_set_debug: unknown flags %d", flags);
-    if ((flags & 1) && !s->F)
+    if (flags & ~(LF_DEBUG_FORCES | LF_DEBUG_SCAN)) return fail(LF_ERR_INVALID, "lf_set_debug: unknown flags %d", flags);
+    if ((flags & LF_DEBUG_FORCES) && !s->F)
         if (int rc = dalloc(&s->F, 4 * (size_t)s->n)) return rc;
     s->debug = flags;
     return LF_OK;

@@ -105,9 +105,12 @@ class Scene:
         return c.value
 
     # -- inspection ---------------------------------------------------------
-    def set_debug(self, forces: bool = True, no_cull: bool = False) -> None:
-        """forces: keep the four force components; no_cull: walk all 27 stencil cells."""
-        check(lib.lf_set_debug(self._h, (1 if forces else 0) | (2 if no_cull else 0)), "lf_set_debug")
+    def set_debug(self, forces: bool = True, scan: bool = False) -> None:
+        """forces: keep the four force components; scan: test every stencil
+        slot in the normals, forces and XSPH passes instead of walking the
+        density pass's neighbour masks."""
+        check(lib.lf_set_debug(self._h, (L.DEBUG_FORCES if forces else 0) | (L.DEBUG_SCAN if scan else 0)),
+              "lf_set_debug")
 
     def field(self, which: int) -> np.ndarray:
         n, nc = self.n, self.params.ncells

@@ -133,6 +133,21 @@ def test_replay_double_dam_preset(gpu, oracle_omp):
         compare_substep(g, o, float(np.float32(cfg.params.particle_size)), f"double dam from step {step}")
 
 
+def test_replay_dense_rows(gpu, oracle_omp):
+    """A block at h/4 spacing: stencil rows of more than 64 slots (scanned in
+    full) inside it, mask-walked rows of at most 64 slots at its faces."""
+    cfg = configs.dense_block()
+    states = gpu_states(gpu, cfg, [0, 5])
+    for step, (pos, vel, acc) in states.items():
+        g = gpu_substep(gpu, cfg, pos, vel, acc)
+        o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
+        cells = np.diff(o["start"].astype(np.int64))[:-1].reshape(25, 25, 50)
+        rows = cells[:, :, :-2] + cells[:, :, 1:-1] + cells[:, :, 2:]
+        occupied = rows[rows > 0]
+        assert occupied.max() > 64 and occupied.min() <= 64, "want stencil rows on both sides of 64 slots"
+        compare_substep(g, o, float(np.float32(cfg.params.particle_size)), f"dense block from step {step}")
+
+
 def test_neighbour_sets_bit_exact(gpu, oracle, c1):
     states = gpu_states(gpu, c1, [0, 50])
     for step, (pos, vel, acc) in states.items():
@@ -233,21 +248,26 @@ def test_full_size_column_c2(gpu, oracle_omp):
     compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c2 replay from step 5")
 
 
-@pytest.mark.parametrize("case", ["c1", "double_dam"])
-def test_cell_skipping_is_bitwise_transparent(gpu, case):
-    """Skipping out-of-reach stencil cells gives bitwise the same trajectory as
-    visiting all 27 (same neighbours, same order)."""
-    cfg = configs.dambreak_c1() if case == "c1" else configs.double_dam(n=4096, res=16)
+@pytest.mark.parametrize("case", ["c1", "double_dam", "dense"])
+def test_neighbour_masks_are_bitwise_transparent(gpu, case):
+    """Walking the density pass's neighbour masks gives bitwise the same
+    trajectory as testing every stencil slot (same neighbours, same order).
+    c1 moves little per substep (XSPH walks the skin masks); the double dam's
+    dt = 2 ms moves particles farther than the skin allows (XSPH scans); the
+    dense block has stencil rows longer than 64 slots (those rows are scanned
+    in every pass)."""
+    cfg = {"c1": configs.dambreak_c1, "double_dam": lambda: configs.double_dam(n=4096, res=16),
+           "dense": configs.dense_block}[case]()
     outs = []
-    for no_cull in (False, True):
+    for scan in (False, True):
         sc = gpu.Scene(cfg.params, cfg.n)
         try:
             sc.upload(cfg.pos, cfg.vel)
-            sc.set_debug(forces=False, no_cull=no_cull)
+            sc.set_debug(forces=False, scan=scan)
             sc.run(cfg.dt, 20)
             sc.sync()
             outs.append(sc.download(L.LF_ORDER_ID))
         finally:
             sc.close()
     for k in ("pos", "vel", "acc", "colour"):
-        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs with and without cell skipping"
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs between mask walks and full scans"
