@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample duration")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, launch the plain copy kernel once over MIB MiB (calibrates byte counters)")
     ap.add_argument("--scan", action="store_true",
                     help="test every stencil slot in every pass instead of walking neighbour masks")
     return ap.parse_args()
@@ -151,6 +153,10 @@ def main():
         sc.step(cfg.dt, 1)
         sc.sync()
         lat.append(1e3 * (time.perf_counter() - s0))
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        ms = sc.copy_probe(nbytes, repeats=1)
+        log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written in {ms:.4f} ms")
     sc.close()
 
     traffic = None
