@@ -87,6 +87,7 @@ int lf_params_check(const lf_params *p);
 
 /* ---- scene lifetime (FluidSystem::init / clear / ~FluidSystem,
  *      fluidsystem.cu:18-52,196-265) ------------------------------------ */
+/* num_particles must be in [1, 2^28). */
 int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out);
 int lf_destroy(lf_scene *s);
 int lf_num_particles(const lf_scene *s, uint32_t *n);
@@ -135,10 +136,12 @@ int lf_get_field(lf_scene *s, int field, void *dst, size_t bytes);
 /* Debug flags: LF_DEBUG_FORCES also writes the four force components
  * (LF_FIELD_FORCES) each substep; LF_DEBUG_SCAN makes the normals, forces and
  * XSPH passes test every slot of the 27-cell stencil instead of walking the
- * neighbour masks the density pass records (results are bitwise the same
- * either way). */
+ * neighbour masks the density pass records; LF_DEBUG_NORMALS_PASS runs the
+ * normals as their own pass also when surface tension is zero (otherwise the
+ * force pass computes them).  Results are bitwise the same either way. */
 #define LF_DEBUG_FORCES 1
 #define LF_DEBUG_SCAN 2
+#define LF_DEBUG_NORMALS_PASS 4
 int lf_set_debug(lf_scene *s, int flags);
 /* Neighbour sets of the density pass for the current state (same FP32
  * predicate as the kernels), by particle id: counts[n], and when list is not

@@ -36,6 +36,7 @@ FIELD_SORTED_POS = 7
 
 DEBUG_FORCES = 1
 DEBUG_SCAN = 2
+DEBUG_NORMALS_PASS = 4
 
 KERNELS = ("hash", "scan", "scatter", "gather", "density", "normals", "forces", "xsph")
 K_INDEX = {k: i for i, k in enumerate(KERNELS)}

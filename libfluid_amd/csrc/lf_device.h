@@ -8,6 +8,7 @@
 namespace lf {
 
 constexpr int kBlock = 256;             // threads per workgroup for every particle kernel
+constexpr uint32_t kMaxParticles = 1u << 28;  // largest scene: 16-byte records addressed with 32-bit offsets
 constexpr uint32_t kRankFixMax = 4096;  // largest bucket the in-cell id ranking handles
 constexpr int kScanPerThread = 16;      // cell-count scan: entries per thread
 constexpr int kScanTile = kBlock * kScanPerThread;
@@ -140,6 +141,14 @@ __device__ __forceinline__ float3 density_colour(const Consts &c, float rho) {
 }
 
 // Fast reciprocal / reciprocal square root (v_rcp_f32 / v_rsq_f32, ~1 ulp).
+// Element j of a particle array through a 32-bit byte offset (particle
+// counts are capped at kMaxParticles, so 16 j < 2^32): the loads then use a
+// scalar base plus a 32-bit vector offset instead of 64-bit address math.
+template <class T>
+__device__ __forceinline__ T at32(const T *__restrict__ base, uint32_t j) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + j * (uint32_t)sizeof(T));
+}
+
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 

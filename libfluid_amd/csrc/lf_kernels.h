@@ -29,8 +29,10 @@ void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const ui
                     float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move);
 void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uint32_t *start, RowMasks M,
                     uint32_t n, float4 *N);
-void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Q, const float *Pr,
-                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, const float4 *N,
+// own_normal: compute the normals inside the force walk (only valid when no
+// force term reads a neighbour's normal, i.e. zero surface tension).
+void launch_forces(const Launch &L, const Consts &c, bool own_normal, const float4 *Ps, const float4 *Q,
+                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
                    const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
                    float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move);
 void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,

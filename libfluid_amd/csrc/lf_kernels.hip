@@ -213,10 +213,12 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // are neighbours (bit k = slot b[r] + k), once for radius h and once for the
 // skin radius.  walk_masks visits the set bits in increasing slot order, the
 // same order as a full scan; a row longer than 64 slots has no mask and is
-// scanned in full.  With use == false every row is scanned.
-template <class F>
+// scanned in full.  With use == false every row is scanned.  Each step takes
+// two set bits and issues both candidates' loads (ld) before using them
+// (fn, in slot order), so two gathers per lane are in flight at a time.
+template <class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
-                                           uint32_t n, uint32_t i, F &&f) {
+                                           uint32_t n, uint32_t i, Ld &&ld, Fn &&fn) {
     uint64_t mk[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
@@ -231,13 +233,19 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
         if (use && e[r] - b[r] <= 64u) {
             uint64_t m = mk[r];
             while (m) {
-                const uint32_t k = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                const uint32_t j1 = b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u;
                 m &= m - 1ull;
-                f(b[r] + k);
+                const bool two = m != 0ull;
+                const uint32_t j2 = two ? b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u : j1;
+                m &= m - 1ull;  // stays 0 when it was 0
+                const auto a1 = ld(j1);
+                const auto a2 = ld(j2);
+                fn(a1, j1);
+                if (two) fn(a2, j2);
             }
         } else {
             for (uint32_t j = b[r]; j < e[r]; ++j)
-                if (j != i) f(j);
+                if (j != i) fn(ld(j), j);
         }
     }
 }
@@ -262,30 +270,42 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
     uint32_t b[9], e[9];
     rows9(c, start, pi, b, e);
     float S = 0.0f;
+    // Branch-free candidate tests: a candidate outside h (or the particle
+    // itself) adds an exact zero to S.  The slot offset k runs in step across
+    // the wave, so the mask bit (1 << k) is a scalar; slots 32..63 of a long
+    // row go to the high words.
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        uint64_t m = 0ull, ms = 0ull;
-        for (uint32_t j = b[r]; j < e[r]; ++j) {
-            const float rr = dist2(pi, Ps[j]);
-            const uint32_t k = j - b[r];
-            const uint64_t bit = k < 64u ? (1ull << k) : 0ull;
-            if (j != i) {
-                if (rr <= c.h2) {
-                    const float t = c.h2 - rr;
-                    S += t * t * t;
-                    m |= bit;
-                }
-                if (rr <= c.h2s) ms |= bit;
-            }
+        const uint32_t bb = b[r], len = e[r] - b[r];
+        const uint32_t kself = i - bb;  // >= len (wrapped) unless i is in this row
+        uint32_t m0 = 0u, s0 = 0u, m1 = 0u, s1 = 0u;
+        const uint32_t l0 = len < 32u ? len : 32u;
+        for (uint32_t k = 0; k < l0; ++k) {
+            const float rr = dist2(pi, at32(Ps, bb + k));
+            const bool other = k != kself;
+            const bool in = rr <= c.h2 && other, ins = rr <= c.h2s && other;
+            const float t = in ? c.h2 - rr : 0.0f;
+            S += t * t * t;
+            m0 |= (in ? 1u : 0u) << k;
+            s0 |= (ins ? 1u : 0u) << k;
         }
-        const uint32_t len = e[r] - b[r];
+        for (uint32_t k = 32u; k < len; ++k) {  // rows of more than 32 slots
+            const float rr = dist2(pi, at32(Ps, bb + k));
+            const bool other = k != kself;
+            const bool in = rr <= c.h2 && other, ins = rr <= c.h2s && other;
+            const float t = in ? c.h2 - rr : 0.0f;
+            S += t * t * t;
+            const uint32_t bit = k < 64u ? 1u << (k & 31u) : 0u;
+            m1 |= in ? bit : 0u;
+            s1 |= ins ? bit : 0u;
+        }
         const size_t at = (size_t)r * n + i;
         if (len <= 32u) {
-            M.w32[at] = (uint32_t)m;
-            MS.w32[at] = (uint32_t)ms;
+            M.w32[at] = m0;
+            MS.w32[at] = s0;
         } else if (len <= 64u) {
-            M.w64[at] = m;
-            MS.w64[at] = ms;
+            M.w64[at] = (uint64_t)m1 << 32 | m0;
+            MS.w64[at] = (uint64_t)s1 << 32 | s0;
         }
     }
     S *= c.mC6;  // sum of m * poly6
@@ -311,7 +331,19 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
 // 6. surface normals (computeNormals, fluidphysics.cu:154-210):
 // n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
 // gradPoly6's factor r.  Neighbours with density <= FLT_MIN (Q.w < 0) or at
-// r^2 <= FLT_MIN contribute zero.
+// r^2 <= FLT_MIN contribute zero.  normal_term is shared by k_normals and by
+// k_forces<true>, which computes the normals inside the force walk when no
+// force term reads a neighbour's normal; both give the same bits.
+__device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, float &sx, float &sy,
+                                            float &sz) {
+    float t = fmaxf(c.h2 - rr, 0.0f);
+    t = rr > c.eps ? t : 0.0f;
+    const float w = t * t * fmaxf(q.w, 0.0f);
+    sx += w * (pi.x - q.x);
+    sy += w * (pi.y - q.y);
+    sz += w * (pi.z - q.z);
+}
+
 __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Q,
                                                     const uint32_t *__restrict__ start, RowMasks M,
                                                     uint32_t n, float4 *__restrict__ N) {
@@ -325,16 +357,9 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
     uint32_t b[9], e[9];
     rows9(c, start, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) {
-        const float4 q = Q[j];
-        const float rr = dist2(pi, q);
-        float t = fmaxf(c.h2 - rr, 0.0f);
-        t = rr > c.eps ? t : 0.0f;
-        const float w = t * t * fmaxf(q.w, 0.0f);
-        sx += w * (pi.x - q.x);
-        sy += w * (pi.y - q.y);
-        sz += w * (pi.z - q.z);
-    });
+    walk_masks(
+        b, e, M, c.use_masks, n, i, [&](uint32_t j) { return at32(Q, j); },
+        [&](const float4 &q, uint32_t) { normal_term(c, pi, q, dist2(pi, q), sx, sy, sz); });
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
 }
 
@@ -344,11 +369,14 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // still integrated.  Fdbg (may be null) receives the four force components.
 // Writes X = (integrated position, density) for the XSPH pass and sets
 // far_move when a particle moved farther than the XSPH skin masks allow.
+// kOwnNormal: no surface tension, so no neighbour's normal is read; the
+// walk computes this particle's normal itself and writes it to N.
+template <bool kOwnNormal>
 __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
                                                    const float4 *__restrict__ Q, const float *__restrict__ Pr,
                                                    const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                    const float4 *__restrict__ Ls, const float4 *__restrict__ D,
-                                                   const float4 *__restrict__ N, const uint32_t *__restrict__ start,
+                                                   float4 *__restrict__ N, const uint32_t *__restrict__ start,
                                                    RowMasks M, uint32_t n,
                                                    float4 *__restrict__ Pout, float4 *__restrict__ Vout,
                                                    float4 *__restrict__ Aout, float4 *__restrict__ Lout,
@@ -360,18 +388,25 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
         const float4 di = D[i];
         float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
         if (i < start[c.ncells]) {
-            const float4 ni = N[i];
+            const float4 ni = kOwnNormal ? make_float4(0.f, 0.f, 0.f, 0.f) : N[i];
             const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
             const float iri2 = iri * iri;
             float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
             float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
             float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
+            float nx = 0.f, ny = 0.f, nz = 0.f;  // own normal (kOwnNormal)
             uint32_t b[9], e[9];
             rows9(c, start, pi, b, e);
-            walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) {
-                const float4 q = Q[j];
-                const float pj = Pr[j];
+            struct Cand {
+                float4 q;
+                float p;
+            };
+            walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) { return Cand{at32(Q, j), at32(Pr, j)}; },
+                       [&](const Cand &cj, uint32_t j) {
+                const float4 q = cj.q;
+                const float pj = cj.p;
                 const float rr = dist2(pi, q);
+                if (kOwnNormal) normal_term(c, pi, q, rr, nx, ny, nz);
                 if (!(rr <= c.h2 && rr > c.eps)) return;
                 if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
                 const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
@@ -386,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                 Py += cp * ry;
                 Pz += cp * rz;
                 if (c.has_visc) {
-                    const float4 vj = Vs[j];
-                    const float4 dj = D[j];
+                    const float4 vj = at32(Vs, j);
+                    const float4 dj = at32(D, j);
                     const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
                     if (ux < 0.0f) {
                         const float mu = c.h * ux * frcp(rr + c.visc_eps);
@@ -398,8 +433,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                         Qz += cv * rz;
                     }
                 }
-                if (c.has_tension) {
-                    const float4 nj = N[j];
+                if (!kOwnNormal && c.has_tension) {
+                    const float4 nj = at32(N, j);
                     const float hr3r3 = hr * hr * hr * r * r * r;
                     const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
                     const float ch = -c.m2 * coh * inv_r;
@@ -408,6 +443,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                     Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
                 }
             });
+            if (kOwnNormal) N[i] = make_float4(c.hmCg6 * nx, c.hmCg6 * ny, c.hmCg6 * nz, 0.0f);
             fP = make_float3(Px, Py, Pz);
             fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
             // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
@@ -424,6 +460,8 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                     fA = make_float3((float)(s * nbv[0]), (float)(s * nbv[1]), (float)(s * nbv[2]));
                 }
             }
+        } else if (kOwnNormal) {
+            N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         if (Fdbg) {
             Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
@@ -514,20 +552,23 @@ __global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restr
     uint32_t b[9], e[9];
     rows9(c, start, Ps[i], b, e);
     float sx = 0.f, sy = 0.f, sz = 0.f;
-    walk_masks(b, e, MS, masks_ok, n, i, [&](uint32_t j) {
-        const float4 q = X[j];
+    // Both loads of a candidate are issued together; a candidate outside h
+    // (or with rho_i + rho_j <= FLT_MIN) adds exact zeros.
+    struct Cand {
+        float4 x, v;
+    };
+    walk_masks(b, e, MS, masks_ok, n, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
+               [&](const Cand &cj, uint32_t) {
+        const float4 q = cj.x;
+        const float4 vj = cj.v;
         const float rr = dist2(xi, q);
-        if (rr <= c.h2) {
-            const float sr = rho_i + q.w;
-            if (sr > c.eps) {
-                const float t = c.h2 - rr;
-                const float w = t * t * t * frcp(sr);
-                const float4 vj = Vh[j];
-                sx += w * (vj.x - vi.x);
-                sy += w * (vj.y - vi.y);
-                sz += w * (vj.z - vi.z);
-            }
-        }
+        const float sr = rho_i + q.w;
+        const bool in = rr <= c.h2 && sr > c.eps;
+        const float t = in ? c.h2 - rr : 0.0f;
+        const float w = in ? t * t * t * frcp(sr) : 0.0f;
+        sx += w * (vj.x - vi.x);
+        sy += w * (vj.y - vi.y);
+        sz += w * (vj.z - vi.z);
     });
     Vout[i] = make_float4(vi.x + c.xsph_k * sx, vi.y + c.xsph_k * sy, vi.z + c.xsph_k * sz, 0.0f);
 }
@@ -683,13 +724,17 @@ void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uin
     if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Q, start, M, n, N);
 }
 
-void launch_forces(const Launch &L, const Consts &c, const float4 *Ps, const float4 *Q, const float *Pr,
-                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, const float4 *N,
+void launch_forces(const Launch &L, const Consts &c, bool own_normal, const float4 *Ps, const float4 *Q,
+                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
                    const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
                    float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move) {
-    if (n)
-        LF_EXT(L, k_forces, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout, Aout,
-               Lout, X, Fdbg, far_move);
+    if (!n) return;
+    if (own_normal)
+        LF_EXT(L, k_forces<true>, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout,
+               Aout, Lout, X, Fdbg, far_move);
+    else
+        LF_EXT(L, k_forces<false>, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout,
+               Aout, Lout, X, Fdbg, far_move);
 }
 
 void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,

@@ -219,10 +219,16 @@ void enqueue_substep(lf_scene *s, const lf::Consts &c) {
     timed(s, LF_K_DENSITY, [&](const lf::Launch &L) {
         lf::launch_density(L, c, s->P1, s->start, s->n, s->D, s->Q, s->Pr, s->M, s->MS, s->far_move);
     });
-    timed(s, LF_K_NORMALS, [&](const lf::Launch &L) { lf::launch_normals(L, c, s->Q, s->start, s->M, s->n, s->N); });
+    // Normals feed only the surface-tension term; without it the force walk
+    // computes them itself (same bits) and the separate pass is not needed.
+    const bool own_normal = !c.has_tension && !(s->debug & LF_DEBUG_NORMALS_PASS);
+    if (!own_normal)
+        timed(s, LF_K_NORMALS,
+              [&](const lf::Launch &L) { lf::launch_normals(L, c, s->Q, s->start, s->M, s->n, s->N); });
     timed(s, LF_K_FORCES, [&](const lf::Launch &L) {
-        lf::launch_forces(L, c, s->P1, s->Q, s->Pr, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->M, s->n, s->P0,
-                          s->V2, s->A0, s->L0, s->X, (s->debug & LF_DEBUG_FORCES) ? s->F : nullptr, s->far_move);
+        lf::launch_forces(L, c, own_normal, s->P1, s->Q, s->Pr, s->V1, s->A1, s->L1, s->D, s->N, s->start, s->M,
+                          s->n, s->P0, s->V2, s->A0, s->L0, s->X, (s->debug & LF_DEBUG_FORCES) ? s->F : nullptr,
+                          s->far_move);
     });
     timed(s, LF_K_XSPH, [&](const lf::Launch &L) {
         lf::launch_xsph(L, c, s->P1, s->X, s->V2, s->start, s->MS, s->far_move, s->n, s->V0);
@@ -267,6 +273,9 @@ int lf_create(const lf_params *p, uint32_t num_particles, lf_scene **out) {
     *out = nullptr;
     if (int rc = lf_params_check(p)) return rc;
     if (num_particles == 0) return fail(LF_ERR_INVALID, "lf_create: zero particles");
+    if (num_particles >= lf::kMaxParticles)
+        return fail(LF_ERR_INVALID, "lf_create: %u particles exceed the limit of %u per scene", num_particles,
+                    lf::kMaxParticles - 1u);
     lf_scene *s = new (std::nothrow) lf_scene();
     if (!s) return fail(LF_ERR_ALLOC, "lf_create: out of host memory");
     s->params = *p;
@@ -458,7 +467,7 @@ int lf_step_count(const lf_scene *s, uint64_t *count) {
 
 int lf_set_debug(lf_scene *s, int flags) {
     if (!s) return fail(LF_ERR_INVALID, "null scene");
-    if (flags & ~(LF_DEBUG_FORCES | LF_DEBUG_SCAN)) return fail(LF_ERR_INVALID, "lf_set_debug: unknown flags %d", flags);
+    if (flags & ~(LF_DEBUG_FORCES | LF_DEBUG_SCAN | LF_DEBUG_NORMALS_PASS)) return fail(LF_ERR_INVALID, "lf_set_debug: unknown flags %d", flags);
     if ((flags & LF_DEBUG_FORCES) && !s->F)
         if (int rc = dalloc(&s->F, 4 * (size_t)s->n)) return rc;
     s->debug = flags;

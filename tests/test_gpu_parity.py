@@ -271,3 +271,24 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
             sc.close()
     for k in ("pos", "vel", "acc", "colour"):
         assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs between mask walks and full scans"
+
+
+def test_normals_in_force_walk_match_separate_pass(gpu):
+    """Without surface tension the force walk computes the normals itself;
+    they and the trajectory are bitwise those of the separate normals pass."""
+    cfg = configs.dambreak_c1()
+    outs = []
+    for separate in (False, True):
+        sc = gpu.Scene(cfg.params, cfg.n)
+        try:
+            sc.upload(cfg.pos, cfg.vel)
+            sc.set_debug(forces=False, normals_pass=separate)
+            sc.run(cfg.dt, 10)
+            sc.sync()
+            d = sc.download(L.LF_ORDER_ID)
+            d["normal"] = sc.field(L.FIELD_NORMAL)
+            outs.append(d)
+        finally:
+            sc.close()
+    for k in ("pos", "vel", "acc", "colour", "normal"):
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{k} differs between fused and separate normals"
