@@ -16,6 +16,12 @@ region, against the 8 TB/s HBM peak.
 
 cpu_baseline: the FP64 CPU oracle (OpenMP build) timed on a bounded prefix of
 the same workload on this host.
+
+batched: the 256-scene workload (particle counts log-uniform in [2048, 65536],
+own boxes and gravity directions) stepped as one batch (autotuned block
+sizes, one captured hipGraph per substep) against the same scenes stepped one
+after another, each alone, for the same number of substeps; the final states
+of the two runs are compared bit for bit.
 """
 from __future__ import annotations
 
@@ -52,9 +58,74 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU sample duration")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, launch the plain copy kernel once over MIB MiB (calibrates byte counters)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene measurement")
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--scan", action="store_true",
                     help="test every stencil slot in every pass instead of walking neighbour masks")
     return ap.parse_args()
+
+
+def batched(args) -> dict:
+    """Batched versus sequential stepping of the multi-scene workload."""
+    import libfluid_amd as lf
+    from libfluid_amd import _lib as L
+    from libfluid_amd import configs
+
+    bc = configs.batched_c4(scenes=args.batch_scenes)
+    K, W = args.batch_steps, 2
+    slots = sum((s.n + 511) // 512 * 512 for s in bc.scenes)
+    cells = sum(s.params.ncells + 1 for s in bc.scenes)
+    log(f"[bench] batch: {len(bc.scenes)} scenes, {bc.n} particles, {bc.cells} cells, {K} steps")
+    b = lf.Batch(slots, cells, len(bc.scenes))
+    ids = [b.add(s.params, s.n, s.dt) for s in bc.scenes]
+    for i, s in zip(ids, bc.scenes):
+        b.upload(i, s.pos, s.vel)
+    b.step(W)
+    b.sync()
+    tune_ms = b.autotune(3)
+    b.step(1)  # capture the graph with the tuned shapes
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(K)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    final = [b.download(i) for i in ids]
+    tuning = b.tuning()
+    b.close()
+    log(f"[bench] batch: {1e3 * t_batch / K:.3f} ms/step, {bc.n * K / t_batch:.4e} particle-steps/s, tuning {tuning}")
+
+    # Sequential: the same scenes, each alone, W + 1 + K substeps.
+    t_seq = 0.0
+    same = True
+    for j, s in enumerate(bc.scenes):
+        sc = lf.Scene(s.params, s.n)
+        sc.upload(s.pos, s.vel)
+        sc.run(s.dt, W + 1)
+        sc.sync()
+        t0 = time.perf_counter()
+        sc.run(s.dt, K)
+        sc.sync()
+        t_seq += time.perf_counter() - t0
+        d = sc.download(L.LF_ORDER_ID)
+        same &= all(np.array_equal(d[k], final[j][k]) for k in ("pos", "vel", "acc"))
+        sc.close()
+    log(f"[bench] sequential: {1e3 * t_seq / K:.3f} ms per step of all scenes, bitwise equal: {same}")
+    value = bc.n * K / t_batch
+    seq = bc.n * K / t_seq
+    return {
+        "workload": f"{len(bc.scenes)} scenes, {bc.n} particles, {bc.cells} cells; counts log-uniform in "
+                    f"[2048, 65536], own box and gravity direction each, dt=1e-4",
+        "scenes": len(bc.scenes), "particles": bc.n, "steps": K,
+        "value": value, "unit": "particle-steps/s",
+        "scene_steps_per_s": len(bc.scenes) * K / t_batch,
+        "ms_per_step": 1e3 * t_batch / K,
+        "sequential_value": seq, "sequential_ms_per_step": 1e3 * t_seq / K,
+        "speedup_vs_sequential": value / seq,
+        "bitwise_equal_to_sequential": bool(same),
+        "tuning": list(tuning),
+        "autotune_ms": np.round(tune_ms, 4).tolist(),
+    }
 
 
 def cpu_baseline(cfg, target_s: float) -> dict:
@@ -146,6 +217,17 @@ def main():
     sc.profile([])
     kernels_ms = {k: (ms / c if c else None) for k, (c, ms) in per.items()}
 
+    # Same scene, no profiling: each substep replays the captured hipGraph.
+    sc.step(cfg.dt, 1)
+    sc.sync()
+    ng = min(200, args.steps)
+    t0 = time.perf_counter()
+    for _ in range(ng):
+        sc.step(cfg.dt, 1)
+    sc.sync()
+    graph_ms = 1e3 * (time.perf_counter() - t0) / ng
+    log(f"[bench] graph replay: {graph_ms:.4f} ms/step")
+
     # Per-step latency (each step synchronised).
     lat = []
     for _ in range(args.latency_steps):
@@ -199,10 +281,12 @@ def main():
         },
         "step_roofline_frac": (STEP_BYTES_PER_PARTICLE + 12.0 * cfg.params.ncells / n) * value / HBM_PEAK,
         "kernels_ms": kernels_ms,
+        "graph_ms_per_step": graph_ms,
         "p50_step_ms": float(np.percentile(lat, 50)),
         "p99_step_ms": float(np.percentile(lat, 99)),
         "vs_baseline_ref": "README.md:2 of the reference, 1M particles at 77 FPS on an RTX 2080 = 7.7e7 particle-steps/s",
     }
+    out["batched"] = None if args.no_batch else batched(args)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)

@@ -32,7 +32,8 @@ enum lf_status {
     LF_ERR_HIP = -2,      /* HIP runtime failure */
     LF_ERR_ALLOC = -3,    /* device or host allocation failed */
     LF_ERR_STATE = -4,    /* call not valid in the scene's current state */
-    LF_ERR_OVERFLOW = -5  /* a cell held more particles than the in-cell ranking supports */
+    LF_ERR_OVERFLOW = -5, /* a cell held more particles than the in-cell ranking supports */
+    LF_ERR_FULL = -6      /* a batch has no room for another scene */
 };
 
 /* Particle order for lf_download. */
@@ -165,6 +166,64 @@ int lf_profile_read(lf_scene *s, uint64_t *launches, double *ms);
  * scene-owned scratch; used to calibrate hardware byte counters.  Returns
  * the event-timed duration in ms. */
 int lf_copy_probe(lf_scene *s, size_t bytes, int repeats, double *ms);
+
+/* ---- launch shapes (no counterpart in the reference, whose launch shape is
+ *      derived from the largest cell occupancy every substep,
+ *      fluidsystem.cu:319-323) ------------------------------------------- */
+/* Workgroup sizes of the density, normals, forces and XSPH passes, each 128,
+ * 256 or 512 (default 256).  Results do not depend on them. */
+int lf_set_tuning(lf_scene *s, const int block_sizes[4]);
+int lf_get_tuning(const lf_scene *s, int block_sizes[4]);
+/* Times each block size on the scene's current state (`reps` substeps each,
+ * state restored) and keeps the fastest per pass.  ms (may be NULL) receives
+ * 12 averages: pass-major, block sizes 128, 256, 512.  Needs one lf_step first
+ * (the substep length). */
+int lf_autotune(lf_scene *s, uint32_t reps, double *ms);
+/* On (default): lf_step replays one captured hipGraph per substep; off: every
+ * kernel is launched individually.  Profiling (lf_profile_enable) always
+ * launches individually.  Results are the same either way. */
+int lf_set_graphs(lf_scene *s, int on);
+
+/* ---- batches of independent scenes ------------------------------------
+ * The reference steps one FluidSystem at a time; a batch keeps many scenes
+ * (own parameters, box, grid, gravity, particle count and substep length)
+ * resident in one pool of device memory and advances all active ones with
+ * one launch per stage.  Scenes can be added and removed between steps.
+ * Each scene's results are bitwise those of the same scene in its own
+ * lf_scene.  Scene handles are small integers, reused after removal. */
+typedef struct lf_batch lf_batch;
+/* Device bytes a batch of the given capacity allocates. */
+int lf_batch_footprint(uint32_t max_particles, uint32_t max_cells, uint32_t max_scenes, uint64_t *bytes);
+/* max_particles: total particle slots (each scene takes its count rounded up
+ * to 512); max_cells: total grid cells plus one per scene; max_scenes:
+ * simultaneous scenes. */
+int lf_batch_create(uint32_t max_particles, uint32_t max_cells, uint32_t max_scenes, lf_batch **out);
+int lf_batch_destroy(lf_batch *b);
+/* Admits a scene (state like lf_create: all particles at the origin until
+ * uploaded) with substep length dt; LF_ERR_FULL when it does not fit. */
+int lf_batch_add(lf_batch *b, const lf_params *p, uint32_t num_particles, float dt, uint32_t *scene);
+int lf_batch_remove(lf_batch *b, uint32_t scene);
+int lf_batch_set_params(lf_batch *b, uint32_t scene, const lf_params *p);
+int lf_batch_set_dt(lf_batch *b, uint32_t scene, float dt);
+/* Inactive scenes keep their state but are not stepped. */
+int lf_batch_set_active(lf_batch *b, uint32_t scene, int active);
+/* As lf_upload / lf_download, for one scene. */
+int lf_batch_upload(lf_batch *b, uint32_t scene, uint32_t first, uint32_t count, const float *pos_xyz,
+                    const float *vel_xyz, const float *acc_xyz);
+int lf_batch_download(lf_batch *b, uint32_t scene, int order, float *pos_xyz, float *colour_rgb, float *vel_xyz,
+                      float *acc_xyz, uint32_t *ids);
+/* Enqueues `substeps` substeps of every active scene (each its own dt). */
+int lf_batch_step(lf_batch *b, uint32_t substeps);
+int lf_batch_sync(lf_batch *b);
+int lf_batch_scene_info(const lf_batch *b, uint32_t scene, uint32_t *num_particles, uint64_t *steps, int *active);
+int lf_batch_usage(const lf_batch *b, uint32_t *scenes, uint32_t *slots_used, uint32_t *slot_cap,
+                   uint32_t *buckets_used, uint32_t *bucket_cap);
+int lf_batch_set_tuning(lf_batch *b, const int block_sizes[4]);
+int lf_batch_get_tuning(const lf_batch *b, int block_sizes[4]);
+int lf_batch_autotune(lf_batch *b, uint32_t reps, double *ms);
+int lf_batch_set_graphs(lf_batch *b, int on);
+int lf_batch_profile_enable(lf_batch *b, int mask);
+int lf_batch_profile_read(lf_batch *b, uint64_t *launches, double *ms);
 
 /* ---- host-side helpers (no device needed) ---------------------------- */
 /* createFluidBlock (fluidsystem.cu:66-127): lattice of up to n points in the

@@ -38,6 +38,8 @@ DEBUG_FORCES = 1
 DEBUG_SCAN = 2
 DEBUG_NORMALS_PASS = 4
 
+LF_ERR_FULL = -6
+
 KERNELS = ("hash", "scan", "scatter", "gather", "density", "normals", "forces", "xsph")
 K_INDEX = {k: i for i, k in enumerate(KERNELS)}
 
@@ -121,6 +123,30 @@ def _load():
         ),
         "lf_uniform_points": (C.c_int, [C.c_uint32, f32p, f32p, C.c_uint64, f32p]),
         "lf_write_geo": (C.c_int, [C.c_char_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_float]),
+        "lf_set_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_get_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_autotune": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "lf_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+        "lf_batch_footprint": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_uint64)]),
+        "lf_batch_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_void_p)]),
+        "lf_batch_destroy": (C.c_int, [C.c_void_p]),
+        "lf_batch_add": (C.c_int, [C.c_void_p, P(LfParams), C.c_uint32, C.c_float, u32p]),
+        "lf_batch_remove": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "lf_batch_set_params": (C.c_int, [C.c_void_p, C.c_uint32, P(LfParams)]),
+        "lf_batch_set_dt": (C.c_int, [C.c_void_p, C.c_uint32, C.c_float]),
+        "lf_batch_set_active": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+        "lf_batch_upload": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p, f32p]),
+        "lf_batch_download": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, f32p, f32p, f32p, f32p, u32p]),
+        "lf_batch_step": (C.c_int, [C.c_void_p, C.c_uint32]),
+        "lf_batch_sync": (C.c_int, [C.c_void_p]),
+        "lf_batch_scene_info": (C.c_int, [C.c_void_p, C.c_uint32, u32p, P(C.c_uint64), P(C.c_int)]),
+        "lf_batch_usage": (C.c_int, [C.c_void_p, u32p, u32p, u32p, u32p, u32p]),
+        "lf_batch_set_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_batch_get_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
+        "lf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+        "lf_batch_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+        "lf_batch_profile_read": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -137,7 +163,12 @@ EXPORTED = (
     "lf_destroy lf_num_particles lf_set_params lf_get_params lf_upload lf_download lf_step "
     "lf_sync lf_step_count lf_get_field lf_set_debug lf_neighbours lf_profile_enable "
     "lf_profile_read lf_copy_probe lf_fluid_block lf_jittered_lattice lf_uniform_points "
-    "lf_write_geo"
+    "lf_write_geo lf_set_tuning lf_get_tuning lf_autotune lf_set_graphs "
+    "lf_batch_footprint lf_batch_create lf_batch_destroy lf_batch_add lf_batch_remove "
+    "lf_batch_set_params lf_batch_set_dt lf_batch_set_active lf_batch_upload lf_batch_download "
+    "lf_batch_step lf_batch_sync lf_batch_scene_info lf_batch_usage lf_batch_set_tuning "
+    "lf_batch_get_tuning lf_batch_autotune lf_batch_set_graphs lf_batch_profile_enable "
+    "lf_batch_profile_read"
 ).split()
 
 

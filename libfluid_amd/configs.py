@@ -19,6 +19,10 @@ seeded generators in the library, so every consumer sees identical bytes.
   double_dam    the reference double dam break (test/src/myfluidsystem.cpp:6-30)
                 on the unit cube with res^3 cells, coefficients a/v/s/d of the
                 reference driver (test/src/main.cpp:86-89)
+  batched_c4    256 independent scenes: particle counts log-uniform in
+                [2048, 65536], each a jittered block in the corner of its own
+                box (1.5-2.5x the block per axis) under gravity 9.8 along its
+                own random direction; dt 1e-4 (NumPy PCG64 from the seed)
 """
 from __future__ import annotations
 
@@ -123,6 +127,49 @@ def dense_block(nx: int = 30, ny: int = 30, nz: int = 30, spacing: float = 0.25 
     p.particle_size = spacing / 2
     pos = jittered_lattice(nx, ny, nz, spacing, (0.503, 0.107, 0.111), 0.05 * spacing, seed)
     return Config(f"dense_{nx}x{ny}x{nz}", p, pos, np.zeros_like(pos), 1e-4, 10)
+
+
+@dataclasses.dataclass
+class BatchConfig:
+    name: str
+    scenes: list  # Config per scene
+
+    @property
+    def n(self) -> int:
+        return sum(s.n for s in self.scenes)
+
+    @property
+    def cells(self) -> int:
+        return sum(s.params.ncells for s in self.scenes)
+
+
+def block_scene(n: int, box_factor, gravity_dir, seed: int, name: str = "block") -> Config:
+    """n particles: the first n points of a jittered lattice (x outermost),
+    nearly cubic, in the corner of a box box_factor[a] times the block's extent
+    along each axis, grid ceil(box / h) cells; gravity 9.8 along gravity_dir."""
+    side = int(np.ceil(n ** (1.0 / 3.0)))
+    nx = nz = side
+    ny = int(np.ceil(n / (nx * nz)))
+    ext = (nx * DX, ny * DX, nz * DX)
+    box = [float(np.float32(ext[a] * box_factor[a])) for a in range(3)]
+    grid = [int(np.ceil(box[a] / H)) for a in range(3)]
+    p = _plain_params(box, grid)
+    g = np.asarray(gravity_dir, np.float64)
+    g = 9.8 * g / np.linalg.norm(g)
+    p.gravity[0], p.gravity[1], p.gravity[2] = (float(v) for v in g)
+    pos = jittered_lattice(nx, ny, nz, DX, (0.0, 0.0, 0.0), 0.05 * DX, seed)[:n].copy()
+    return Config(name, p, pos, np.zeros_like(pos), 1e-4, 200)
+
+
+def batched_c4(seed: int = 4, scenes: int = 256, lo: int = 2048, hi: int = 65536) -> BatchConfig:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(scenes):
+        n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        box_factor = rng.uniform(1.5, 2.5, size=3)
+        gdir = rng.normal(size=3)
+        out.append(block_scene(n, box_factor, gdir, seed * 1000 + k, name=f"c4_scene{k}"))
+    return BatchConfig(f"batched_c4_{scenes}", out)
 
 
 CONFIGS = {"c1": dambreak_c1, "c2": column_c2, "c3": spray_c3, "double_dam": double_dam}

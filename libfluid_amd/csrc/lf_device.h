@@ -7,25 +7,33 @@
 
 namespace lf {
 
-constexpr int kBlock = 256;             // threads per workgroup for every particle kernel
-constexpr uint32_t kMaxParticles = 1u << 28;  // largest scene: 16-byte records addressed with 32-bit offsets
-constexpr uint32_t kRankFixMax = 4096;  // largest bucket the in-cell id ranking handles
-constexpr int kScanPerThread = 16;      // cell-count scan: entries per thread
+constexpr int kBlock = 256;                   // threads per workgroup of the fixed-shape kernels
+constexpr uint32_t kChunk = 512;              // particle-slot granularity of scene ranges (largest block)
+constexpr uint32_t kNoScene = 0xFFFFFFFFu;    // chunk owned by no scene
+constexpr uint32_t kMaxParticles = 1u << 28;  // pool slots: 16-byte records addressed with 32-bit offsets
+constexpr uint32_t kRankFixMax = 4096;        // largest bucket the in-cell id ranking handles
+constexpr int kScanPerThread = 16;            // cell-count scan: entries per thread
 constexpr int kScanTile = kBlock * kScanPerThread;
 
-// Kernel-argument block: every constant a substep needs, derived on the host
-// (in double, rounded once to float) from lf_params.  Passed by value, so a
-// launch reads it through the kernarg segment (scalar loads).
-// Neighbour masks of one radius for 9 stencil rows x n particles, row-major
-// (entry r * n + i).  Bit k stands for slot b_r + k of row r.  A row of at
-// most 32 slots keeps its mask in w32, a row of 33..64 slots in w64; longer
-// rows have no mask.
+// Neighbour masks of one radius for 9 stencil rows x nslots particle slots,
+// row-major (entry r * nslots + i).  Bit k stands for slot b_r + k of row r.
+// A row of at most 32 slots keeps its mask in w32, a row of 33..64 slots in
+// w64; longer rows have no mask.
 struct RowMasks {
     uint32_t *w32;
     uint64_t *w64;
 };
 
+// Per-scene constants: everything a substep needs, derived on the host (in
+// double, rounded once to float) from lf_params, plus where the scene lives
+// in the pool.  A pool holds one table of these in device memory; each
+// workgroup serves one scene and reads its entry with scalar loads.
 struct Consts {
+    // place in the pool: particle slots [first, first + count) hold the
+    // scene's particles (its chunks may hold unused slots after them),
+    // buckets [cell_base, cell_base + ncells + 1) its cells and its
+    // out-of-grid bucket; `scene` indexes per-scene flags
+    uint32_t first, count, cell_base, scene;
     // grid (FluidParamData m_res/m_res2/m_numCells, generalised per axis)
     float inv_cell[3];
     int gx, gy, gz;
@@ -41,12 +49,14 @@ struct Consts {
     double move_limit;
     int use_masks;
     // density + EOS
-    float mC6;        // m * 315/(64 pi h^9)
+    float mC6;  // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
-    // normals
-    float hmCg6;      // h * m * (-945/(32 pi h^9))
+    // normals: own_normal = the force pass computes this scene's normals
+    // (no surface tension, so no force term reads a neighbour's normal)
+    float hmCg6;  // h * m * (-945/(32 pi h^9))
+    int own_normal;
     // forces
-    float mCgs;       // m * (-45/(pi h^6))
+    float mCgs;  // m * (-45/(pi h^6))
     float nu, sigma, alpha, m2, sigma_m, visc_eps, halfh, coh_off;
     double quarter_inv_psize, alpha_inv_rho0;
     int has_visc, has_tension, has_adhesion;
@@ -54,7 +64,7 @@ struct Consts {
     double g[3], m, dt, dtdt, vlim, vlim2, alim, alim2;
     double vdamp1, psize, h_d, bd_rho0_d;
     // XSPH
-    float xsph_k;     // xsph * 2 m * C6
+    float xsph_k;  // xsph * 2 m * C6
 };
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
@@ -69,8 +79,8 @@ __device__ __forceinline__ float dist2(float4 a, float4 b) {
 // Cell coordinate along one axis: floor(x * res) (fluidutil.cu:15-18).
 __device__ __forceinline__ float cell_coord(float x, float inv_cell) { return floorf(x * inv_cell); }
 
-// Bucket of a position: cell key kx + gx*(ky + gy*kz) (fluidutil.cu:23-39), or
-// ncells for positions outside the grid (also NaN).
+// Scene-local bucket of a position: cell key kx + gx*(ky + gy*kz)
+// (fluidutil.cu:23-39), or ncells for positions outside the grid (also NaN).
 __device__ __forceinline__ uint32_t bucket_of(const Consts &c, float4 p) {
     float fx = cell_coord(p.x, c.inv_cell[0]);
     float fy = cell_coord(p.y, c.inv_cell[1]);
@@ -88,18 +98,20 @@ __device__ __forceinline__ void cell_xyz(const Consts &c, float4 p, int &cx, int
 
 // Visits every slot j of the clipped 3x3x3 cell stencil around (cx,cy,cz).
 // With keys ordered x fastest, the three cells of one x-row are contiguous in
-// the sorted arrays, so the stencil is nine contiguous slot ranges.
+// the sorted arrays, so the stencil is nine contiguous slot ranges.  `start`
+// is the scene's own table (pool table + cell_base); a table entry plus
+// `shift` is a slot.
 template <class F>
-__device__ __forceinline__ void for_stencil(const Consts &c, const uint32_t *__restrict__ start, int cx, int cy,
-                                            int cz, F &&f) {
+__device__ __forceinline__ void for_stencil(const Consts &c, const uint32_t *__restrict__ start, uint32_t shift,
+                                            int cx, int cy, int cz, F &&f) {
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < c.gx - 1 ? cx + 1 : c.gx - 1;
     const int y0 = cy > 0 ? cy - 1 : 0, y1 = cy < c.gy - 1 ? cy + 1 : c.gy - 1;
     const int z0 = cz > 0 ? cz - 1 : 0, z1 = cz < c.gz - 1 ? cz + 1 : c.gz - 1;
     for (int z = z0; z <= z1; ++z) {
         for (int y = y0; y <= y1; ++y) {
             const uint32_t row = (uint32_t)(z * c.gy + y) * (uint32_t)c.gx;
-            const uint32_t b = start[row + (uint32_t)x0];
-            const uint32_t e = start[row + (uint32_t)x1 + 1u];
+            const uint32_t b = start[row + (uint32_t)x0] + shift;
+            const uint32_t e = start[row + (uint32_t)x1 + 1u] + shift;
             for (uint32_t j = b; j < e; ++j) f(j);
         }
     }
@@ -140,15 +152,15 @@ __device__ __forceinline__ float3 density_colour(const Consts &c, float rho) {
     return make_float3(1.0f, 0.0f, 0.0f);
 }
 
-// Fast reciprocal / reciprocal square root (v_rcp_f32 / v_rsq_f32, ~1 ulp).
-// Element j of a particle array through a 32-bit byte offset (particle
-// counts are capped at kMaxParticles, so 16 j < 2^32): the loads then use a
-// scalar base plus a 32-bit vector offset instead of 64-bit address math.
+// Element j of a particle array through a 32-bit byte offset (pool slots are
+// capped at kMaxParticles, so 16 j < 2^32): the loads then use a scalar base
+// plus a 32-bit vector offset instead of 64-bit address math.
 template <class T>
 __device__ __forceinline__ T at32(const T *__restrict__ base, uint32_t j) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + j * (uint32_t)sizeof(T));
 }
 
+// Fast reciprocal / reciprocal square root (v_rcp_f32 / v_rsq_f32, ~1 ulp).
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
@@ -159,6 +171,19 @@ __device__ __forceinline__ uint32_t xcd_block() {
     const uint32_t nb = gridDim.x, b = blockIdx.x;
     const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
     return x * q + (x < r ? x : r) + k;
+}
+
+// First slot of this workgroup (BS threads per workgroup over the pool's
+// slots) and the constants of the scene that owns it, or null for a chunk no
+// scene owns.  Scene ranges are whole chunks and BS divides kChunk, so a
+// workgroup never straddles two scenes and the lookup is uniform.
+template <int BS>
+__device__ __forceinline__ const Consts *block_scene(const Consts *__restrict__ cs,
+                                                     const uint32_t *__restrict__ chunk, uint32_t &slot0) {
+    static_assert(kChunk % BS == 0, "block size must divide the chunk");
+    slot0 = xcd_block() * (uint32_t)BS;
+    const uint32_t s = chunk[slot0 / kChunk];
+    return s == kNoScene ? nullptr : cs + s;
 }
 
 }  // namespace lf

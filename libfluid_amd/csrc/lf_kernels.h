@@ -15,40 +15,56 @@ struct Launch {
     hipEvent_t a, b;
 };
 
-uint32_t scan_tiles(uint32_t m);
+// Device scene table of a pool: cs[s] are the constants of scene s, chunk[c]
+// the scene owning slots [512 c, 512 c + 512) (kNoScene if none); nslots is
+// the pool's slot count (a multiple of 512).  Stepping launches cover all
+// nslots slots.
+struct PoolView {
+    const Consts *cs;
+    const uint32_t *chunk;
+    uint32_t nslots;
+};
 
-void launch_hash(const Launch &L, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
+uint32_t scan_tiles(uint32_t m);
+// Block sizes the neighbour passes are built for (128, 256, 512).
+bool valid_block_size(int bs);
+
+void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts);
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
-void launch_scatter(const Launch &L, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
-                    const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id);
-void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
-                   uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
-void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
+void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
+                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);
+void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
+                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
+void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const uint32_t *start, float4 *D,
                     float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move);
-void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uint32_t *start, RowMasks M,
-                    uint32_t n, float4 *N);
-// own_normal: compute the normals inside the force walk (only valid when no
-// force term reads a neighbour's normal, i.e. zero surface tension).
-void launch_forces(const Launch &L, const Consts &c, bool own_normal, const float4 *Ps, const float4 *Q,
-                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
-                   const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
-                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move);
-void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,
-                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, uint32_t n, float4 *Vout);
+// Only scenes without own_normal are processed.
+void launch_normals(const Launch &L, const PoolView &pv, int bs, const float4 *Q, const uint32_t *start, RowMasks M,
+                    float4 *N);
+void launch_forces(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *Q, const float *Pr,
+                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
+                   const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
+                   float4 *X, float4 *Fdbg, uint32_t *far_move);
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *X, const float4 *Vh,
+                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout);
 
-void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A, float4 *Lw);
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *Lw, float4 *D);
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
-                        const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2);
+// Per-scene helpers: slots [first, first + count) of one scene.
+void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t id0, uint32_t count, float4 *P,
+                   float4 *V, float4 *A, float4 *Lw);
+void launch_init_state(hipStream_t st, uint32_t first, uint32_t count, float4 *P, float4 *V, float4 *A, float4 *Lw,
+                       float4 *D);
+void launch_to_id_order(hipStream_t st, uint32_t first, uint32_t count, const float4 *P, const float4 *V,
+                        const float4 *A, const float4 *Lw, const float4 *D, float4 *P2, float4 *V2, float4 *A2,
+                        float4 *L2, float4 *D2);
+void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, const float4 *P, const float4 *V,
+                 const float4 *A, const float4 *Lw, const float4 *D, float4 *P2, float4 *V2, float4 *A2, float4 *L2,
+                 float4 *D2);
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *D, uint32_t n, int by_id, float *out, uint32_t *ids);
-void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                     uint32_t *cnt);
-void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                    const uint64_t *off, uint32_t *list);
+                   const float4 *D, bool by_id, float *out, uint32_t *ids);
+void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t *cnt);
+void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, const uint64_t *off,
+                    uint32_t *list);
 void launch_copy(const Launch &L, const float4 *src, float4 *dst, size_t n4);
 
 }  // namespace lf

@@ -1,15 +1,26 @@
-// lf_kernels.hip -- CDNA4 (gfx950) kernels of one WCSPH substep.
+// lf_kernels.hip -- CDNA4 (gfx950) kernels of one WCSPH substep over a pool
+// of independent scenes.
+//
+// Pool: every per-particle array spans all particle slots of the pool.  A
+// scene owns whole 512-slot chunks starting at slot `first` (its particles
+// fill [first, first + count)) and a bucket range [cell_base, cell_base +
+// ncells + 1): its grid cells and its out-of-grid bucket.  One exclusive scan
+// runs over all buckets of the pool; a scene places its particles at
+// first + (start[b] - start[cell_base]) + rank, so free slots and buckets
+// between scenes, or scenes left out of a step, need no bookkeeping.  Every
+// pass is one launch over all slots of the pool; a workgroup serves one scene
+// and reads that scene's constants (Consts table) with scalar loads.  A
+// single scene is a pool of one.
 //
 // Layout: every per-particle array is SoA float4 (16 B, one dwordx4 load):
 //   P = (x, y, z, id bits)   V = (vx, vy, vz, 0)   A = (ax, ay, az, 0)
 //   L = low-order position words: the position is the double-float P.xyz + L.xyz
 //   D = (density, pressure, sound speed, 1/density)   N = (normal, 0)
-// The sorted arrays are in (cell key, particle id) order; `start` holds the
-// first slot of every bucket (ncells grid cells + one out-of-grid bucket)
-// plus an end sentinel.
+// Ids are scene-local.  The sorted arrays are in (cell key, particle id)
+// order; `start` holds the first slot of every bucket plus an end sentinel.
 //
 // The density pass tests every candidate of the clipped 27-cell stencil (nine
-// contiguous x-row slot ranges) and records, per row, 64-bit masks of the
+// contiguous x-row slot ranges) and records, per row, 32/64-bit masks of the
 // slots within h and within a skin radius.  The normals and forces passes walk
 // the exact masks, XSPH the skin masks (when the substep's largest move
 // proves them complete), over packed neighbour data; set bits are visited in
@@ -20,6 +31,8 @@
 // are VALU/L1 work over the stencil walk and never use MFMA.
 
 #include <hip/hip_ext.h>
+
+#include <algorithm>
 
 #include "lf_device.h"
 #include "lf_kernels.h"
@@ -32,13 +45,17 @@ namespace lf {
 // inside its bucket.  Particles arrive in the previous cell order, so runs of
 // lanes share a bucket: each run issues one atomic for its length and hands
 // out consecutive ranks.  Counts are zeroed by the previous substep's scan.
-__global__ __launch_bounds__(kBlock) void k_hash(Consts c, const float4 *__restrict__ P, uint32_t n,
-                                                 uint32_t *__restrict__ keys, uint32_t *__restrict__ ranks,
-                                                 uint32_t *__restrict__ counts) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                                 const float4 *__restrict__ P, uint32_t *__restrict__ keys,
+                                                 uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
+    if (!cp) return;
+    const Consts &c = *cp;
+    const uint32_t i = slot0 + threadIdx.x;
     const int lane = threadIdx.x & 63;
-    const bool active = i < n;
-    const uint32_t b = active ? bucket_of(c, P[i]) : 0xFFFFFFFFu;
+    const bool active = i < c.first + c.count;
+    const uint32_t b = active ? c.cell_base + bucket_of(c, P[i]) : 0xFFFFFFFFu;
     const uint32_t prev = __shfl_up(b, 1, 64);
     const bool head = lane == 0 || prev != b;
     const uint64_t heads = __ballot(head);
@@ -57,7 +74,8 @@ __global__ __launch_bounds__(kBlock) void k_hash(Consts c, const float4 *__restr
 
 // ---------------------------------------------------------------------------
 // 2. cell start table: exclusive scan of the bucket counts
-// (thrust::exclusive_scan, fluidsystem.cu:316), three phases.
+// (thrust::exclusive_scan, fluidsystem.cu:316), three phases over all buckets
+// of the pool.
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;
@@ -99,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_sum(const uint32_t *__restrict_
 }
 
 // One workgroup: exclusive scan of the per-tile sums; writes the grand total
-// (the particle count) as the end sentinel start[m].
+// as the end sentinel start[m].
 __global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *__restrict__ bsum, uint32_t nb,
                                                      uint32_t *__restrict__ start, uint32_t m) {
     __shared__ uint32_t sh[kBlock / 64];
@@ -142,14 +160,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ cou
 }
 
 // ---------------------------------------------------------------------------
+// Slot of the first particle of pool bucket b of scene c:
+//   first + (start[b] - start[cell_base]) = start[b] + shift.
+__device__ __forceinline__ uint32_t slot_shift(const Consts &c, const uint32_t *__restrict__ start) {
+    return c.first - start[c.cell_base];
+}
+
 // 3. provisional scatter: slot = bucket start + atomic rank.
-__global__ __launch_bounds__(kBlock) void k_scatter(const float4 *__restrict__ P, const uint32_t *__restrict__ keys,
+__global__ __launch_bounds__(kBlock) void k_scatter(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                                    const float4 *__restrict__ P, const uint32_t *__restrict__ keys,
                                                     const uint32_t *__restrict__ ranks,
-                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    const uint32_t *__restrict__ start,
                                                     uint32_t *__restrict__ slot_src, uint32_t *__restrict__ slot_id) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = start[keys[i]] + ranks[i];
+    uint32_t slot0;
+    const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
+    if (!cp) return;
+    const uint32_t i = slot0 + threadIdx.x;
+    if (i >= cp->first + cp->count) return;
+    const uint32_t s = start[keys[i]] + slot_shift(*cp, start) + ranks[i];
     slot_src[s] = i;
     slot_id[s] = __float_as_uint(P[i].w);
 }
@@ -158,20 +186,24 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const float4 *__restrict__ P
 // Acceleration, fluidsystem.cu:309-313).  The final slot of a particle is
 // its bucket start plus the number of bucket members with a smaller id, so
 // the order is (key, id) whatever order the atomics handed out.
-__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ slot_src,
+__global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                                   const uint32_t *__restrict__ slot_src,
                                                    const uint32_t *__restrict__ slot_id,
                                                    const uint32_t *__restrict__ keys,
                                                    const uint32_t *__restrict__ start, const float4 *__restrict__ P,
                                                    const float4 *__restrict__ V, const float4 *__restrict__ A,
-                                                   const float4 *__restrict__ L, uint32_t n,
-                                                   float4 *__restrict__ Ps, float4 *__restrict__ Vs,
-                                                   float4 *__restrict__ As, float4 *__restrict__ Ls,
-                                                   uint32_t *__restrict__ err) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+                                                   const float4 *__restrict__ L, float4 *__restrict__ Ps,
+                                                   float4 *__restrict__ Vs, float4 *__restrict__ As,
+                                                   float4 *__restrict__ Ls, uint32_t *__restrict__ err) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
+    if (!cp) return;
+    const uint32_t s = slot0 + threadIdx.x;
+    if (s >= cp->first + cp->count) return;
     const uint32_t src = slot_src[s], id = slot_id[s];
     const uint32_t b = keys[src];
-    const uint32_t lo = start[b], hi = start[b + 1];
+    const uint32_t shift = slot_shift(*cp, start);
+    const uint32_t lo = start[b] + shift, hi = start[b + 1] + shift;
     uint32_t f;
     if (hi - lo <= kRankFixMax) {
         uint32_t r = 0;
@@ -192,8 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t *__restrict__ 
 // slot ranges (x-rows, z outer, y inner: the reference's visiting order of
 // cells, fluidphysics.cu:30-53, up to the order inside a row).  rows9 issues
 // the eighteen start-table loads together; rows outside the grid are empty.
-__device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restrict__ start, float4 pc, uint32_t *b,
-                                      uint32_t *e) {
+// `start` is the scene's own table, `shift` its slot_shift.
+__device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restrict__ start, uint32_t shift, float4 pc,
+                                      uint32_t *b, uint32_t *e) {
     int cx, cy, cz;
     cell_xyz(c, pc, cx, cy, cz);
     const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx < c.gx - 1 ? cx + 1 : c.gx - 1;
@@ -202,15 +235,15 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
         const int z = cz + r / 3 - 1, y = cy + r % 3 - 1;
         const bool ok = z >= 0 && z < c.gz && y >= 0 && y < c.gy;
         const uint32_t row = (uint32_t)((ok ? z : cz) * c.gy + (ok ? y : cy)) * (uint32_t)c.gx;
-        const uint32_t bb = start[row + (uint32_t)x0];
-        const uint32_t ee = start[row + (uint32_t)x1 + 1u];
+        const uint32_t bb = start[row + (uint32_t)x0] + shift;
+        const uint32_t ee = start[row + (uint32_t)x1 + 1u] + shift;
         b[r] = bb;
         e[r] = ok ? ee : bb;
     }
 }
 
-// The density pass records, per row, a 64-bit mask of the row's slots that
-// are neighbours (bit k = slot b[r] + k), once for radius h and once for the
+// The density pass records, per row, a mask of the row's slots that are
+// neighbours (bit k = slot b[r] + k), once for radius h and once for the
 // skin radius.  walk_masks visits the set bits in increasing slot order, the
 // same order as a full scan; a row longer than 64 slots has no mask and is
 // scanned in full.  With use == false every row is scanned.  Each step takes
@@ -218,12 +251,12 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // (fn, in slot order), so two gathers per lane are in flight at a time.
 template <class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
-                                           uint32_t n, uint32_t i, Ld &&ld, Fn &&fn) {
+                                           uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
     uint64_t mk[9];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t len = e[r] - b[r];
-        const size_t at = (size_t)r * n + i;
+        const size_t at = (size_t)r * nslots + i;
         mk[r] = 0ull;
         if (use && len <= 32u) mk[r] = mask.w32[at];
         else if (use && len <= 64u) mk[r] = mask.w64[at];
@@ -253,22 +286,29 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 // 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
 // neighbour masks of the substep and the packed neighbour data of the later
 // passes: Q = (x, y, z, 1/density, or -1 when density <= FLT_MIN), Pr =
-// pressure.
-__global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__restrict__ Ps,
-                                                    const uint32_t *__restrict__ start, uint32_t n,
-                                                    float4 *__restrict__ D, float4 *__restrict__ Q,
-                                                    float *__restrict__ Pr, RowMasks M, RowMasks MS,
-                                                    uint32_t *__restrict__ far_move) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *far_move = 0u;  // set by k_forces when the skin is exceeded
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= n) return;
-    if (i >= start[c.ncells]) {  // out-of-grid bucket: never a neighbour, no masks
+// pressure.  Clears the scene's far_move flag.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                                uint32_t nslots, const float4 *__restrict__ Ps,
+                                                const uint32_t *__restrict__ start, float4 *__restrict__ D,
+                                                float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
+                                                RowMasks MS, uint32_t *__restrict__ far_move) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<BS>(cs, chunk, slot0);
+    if (!cp) return;
+    const Consts &c = *cp;
+    const uint32_t i = slot0 + threadIdx.x;
+    if (i == c.first) far_move[c.scene] = 0u;  // set by k_forces when the skin is exceeded
+    if (i >= c.first + c.count) return;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
+    if (i >= st[c.ncells] + shift) {  // out-of-grid bucket: never a neighbour, no masks
         D[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
     const float4 pi = Ps[i];
     uint32_t b[9], e[9];
-    rows9(c, start, pi, b, e);
+    rows9(c, st, shift, pi, b, e);
     float S = 0.0f;
     // Branch-free candidate tests: a candidate outside h (or the particle
     // itself) adds an exact zero to S.  The slot offset k runs in step across
@@ -299,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
             m1 |= in ? bit : 0u;
             s1 |= ins ? bit : 0u;
         }
-        const size_t at = (size_t)r * n + i;
+        const size_t at = (size_t)r * nslots + i;
         if (len <= 32u) {
             M.w32[at] = m0;
             MS.w32[at] = s0;
@@ -321,9 +361,9 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
     const float rho = (S == 0.0f) ? c.rho0 : corrected;
     const float ratio = corrected * c.inv_rho0;
     const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (powf(ratio, c.gamma) - 1.0f);
-    const float cs = c.K * powf(ratio, c.gamma_m1);
+    const float cs_i = c.K * powf(ratio, c.gamma_m1);
     const float ir = 1.0f / rho;
-    D[i] = make_float4(rho, press, cs, ir);
+    D[i] = make_float4(rho, press, cs_i, ir);
     Q[i] = make_float4(pi.x, pi.y, pi.z, rho > c.eps ? ir : -1.0f);
     Pr[i] = press;
 }
@@ -332,8 +372,8 @@ __global__ __launch_bounds__(kBlock) void k_density(Consts c, const float4 *__re
 // n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
 // gradPoly6's factor r.  Neighbours with density <= FLT_MIN (Q.w < 0) or at
 // r^2 <= FLT_MIN contribute zero.  normal_term is shared by k_normals and by
-// k_forces<true>, which computes the normals inside the force walk when no
-// force term reads a neighbour's normal; both give the same bits.
+// k_forces, which computes the normals inside the force walk for scenes
+// without surface tension (own_normal); both give the same bits.
 __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, float &sx, float &sy,
                                             float &sz) {
     float t = fmaxf(c.h2 - rr, 0.0f);
@@ -344,21 +384,29 @@ __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q
     sz += w * (pi.z - q.z);
 }
 
-__global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__restrict__ Q,
-                                                    const uint32_t *__restrict__ start, RowMasks M,
-                                                    uint32_t n, float4 *__restrict__ N) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= n) return;
-    if (i >= start[c.ncells]) {
+template <int BS>
+__global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                                uint32_t nslots, const float4 *__restrict__ Q,
+                                                const uint32_t *__restrict__ start, RowMasks M,
+                                                float4 *__restrict__ N) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<BS>(cs, chunk, slot0);
+    if (!cp || cp->own_normal) return;
+    const Consts &c = *cp;
+    const uint32_t i = slot0 + threadIdx.x;
+    if (i >= c.first + c.count) return;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
+    if (i >= st[c.ncells] + shift) {
         N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         return;
     }
     const float4 pi = Q[i];
     uint32_t b[9], e[9];
-    rows9(c, start, pi, b, e);
+    rows9(c, st, shift, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     walk_masks(
-        b, e, M, c.use_masks, n, i, [&](uint32_t j) { return at32(Q, j); },
+        b, e, M, c.use_masks, nslots, i, [&](uint32_t j) { return at32(Q, j); },
         [&](const float4 &q, uint32_t) { normal_term(c, pi, q, dist2(pi, q), sx, sy, sz); });
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
 }
@@ -367,46 +415,53 @@ __global__ __launch_bounds__(kBlock) void k_normals(Consts c, const float4 *__re
 // leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
-// Writes X = (integrated position, density) for the XSPH pass and sets
-// far_move when a particle moved farther than the XSPH skin masks allow.
-// kOwnNormal: no surface tension, so no neighbour's normal is read; the
-// walk computes this particle's normal itself and writes it to N.
-template <bool kOwnNormal>
-__global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__restrict__ Ps,
-                                                   const float4 *__restrict__ Q, const float *__restrict__ Pr,
-                                                   const float4 *__restrict__ Vs, const float4 *__restrict__ As,
-                                                   const float4 *__restrict__ Ls, const float4 *__restrict__ D,
-                                                   float4 *__restrict__ N, const uint32_t *__restrict__ start,
-                                                   RowMasks M, uint32_t n,
-                                                   float4 *__restrict__ Pout, float4 *__restrict__ Vout,
-                                                   float4 *__restrict__ Aout, float4 *__restrict__ Lout,
-                                                   float4 *__restrict__ X, float4 *__restrict__ Fdbg,
-                                                   uint32_t *__restrict__ far_move) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i < n) {
-        const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
-        const float4 di = D[i];
-        float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
-        if (i < start[c.ncells]) {
-            const float4 ni = kOwnNormal ? make_float4(0.f, 0.f, 0.f, 0.f) : N[i];
-            const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
-            const float iri2 = iri * iri;
-            float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
-            float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
-            float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
-            float nx = 0.f, ny = 0.f, nz = 0.f;  // own normal (kOwnNormal)
-            uint32_t b[9], e[9];
-            rows9(c, start, pi, b, e);
-            struct Cand {
-                float4 q;
-                float p;
-            };
-            walk_masks(b, e, M, c.use_masks, n, i, [&](uint32_t j) { return Cand{at32(Q, j), at32(Pr, j)}; },
-                       [&](const Cand &cj, uint32_t j) {
+// Writes X = (integrated position, density) for the XSPH pass and sets the
+// scene's far_move flag when a particle moved farther than the XSPH skin
+// masks allow.  For own_normal scenes the walk also computes the normals.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                               uint32_t nslots, const float4 *__restrict__ Ps,
+                                               const float4 *__restrict__ Q, const float *__restrict__ Pr,
+                                               const float4 *__restrict__ Vs, const float4 *__restrict__ As,
+                                               const float4 *__restrict__ Ls, const float4 *__restrict__ D,
+                                               float4 *__restrict__ N, const uint32_t *__restrict__ start,
+                                               RowMasks M, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
+                                               float4 *__restrict__ Aout, float4 *__restrict__ Lout,
+                                               float4 *__restrict__ X, float4 *__restrict__ Fdbg,
+                                               uint32_t *__restrict__ far_move) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<BS>(cs, chunk, slot0);
+    if (!cp) return;
+    const Consts &c = *cp;
+    const uint32_t i = slot0 + threadIdx.x;
+    if (i >= c.first + c.count) return;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
+    const bool own_normal = c.own_normal != 0;
+    const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
+    const float4 di = D[i];
+    float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
+    if (i < st[c.ncells] + shift) {
+        const float4 ni = own_normal ? make_float4(0.f, 0.f, 0.f, 0.f) : N[i];
+        const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
+        const float iri2 = iri * iri;
+        float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
+        float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
+        float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
+        float nx = 0.f, ny = 0.f, nz = 0.f;  // own normal
+        uint32_t b[9], e[9];
+        rows9(c, st, shift, pi, b, e);
+        struct Cand {
+            float4 q;
+            float p;
+        };
+        walk_masks(
+            b, e, M, c.use_masks, nslots, i, [&](uint32_t j) { return Cand{at32(Q, j), at32(Pr, j)}; },
+            [&](const Cand &cj, uint32_t j) {
                 const float4 q = cj.q;
                 const float pj = cj.p;
                 const float rr = dist2(pi, q);
-                if (kOwnNormal) normal_term(c, pi, q, rr, nx, ny, nz);
+                if (own_normal) normal_term(c, pi, q, rr, nx, ny, nz);
                 if (!(rr <= c.h2 && rr > c.eps)) return;
                 if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
                 const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
@@ -416,10 +471,10 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                 const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
                 const float irj = q.w;
                 const float mid = pj * iri2 + p_i * (irj * irj);
-                const float cp = cm * mid;
-                Px += cp * rx;
-                Py += cp * ry;
-                Pz += cp * rz;
+                const float cpr = cm * mid;
+                Px += cpr * rx;
+                Py += cpr * ry;
+                Pz += cpr * rz;
                 if (c.has_visc) {
                     const float4 vj = at32(Vs, j);
                     const float4 dj = at32(D, j);
@@ -433,7 +488,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                         Qz += cv * rz;
                     }
                 }
-                if (!kOwnNormal && c.has_tension) {
+                if (!own_normal && c.has_tension) {
                     const float4 nj = at32(N, j);
                     const float hr3r3 = hr * hr * hr * r * r * r;
                     const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
@@ -443,87 +498,86 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
                     Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
                 }
             });
-            if (kOwnNormal) N[i] = make_float4(c.hmCg6 * nx, c.hmCg6 * ny, c.hmCg6 * nz, 0.0f);
-            fP = make_float3(Px, Py, Pz);
-            fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
-            // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
-            const float ts = c.sigma * c.rho0 * iri;
-            fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
-            if (c.has_adhesion) {
-                // Adhesion toward the walls (fluidphysics.cu:330-350), FP64.
-                const double z = -fmin(box_sdf(c, pi.x, pi.y, pi.z), 0.0) * c.quarter_inv_psize;
-                if (z < 1.0) {
-                    double nbv[3];
-                    box_normal(c, pi.x, pi.y, pi.z, nbv);
-                    const double oz = 1.0 - z;
-                    const double s = c.alpha_inv_rho0 * (double)rho_i / (oz * oz * oz);
-                    fA = make_float3((float)(s * nbv[0]), (float)(s * nbv[1]), (float)(s * nbv[2]));
-                }
+        if (own_normal) N[i] = make_float4(c.hmCg6 * nx, c.hmCg6 * ny, c.hmCg6 * nz, 0.0f);
+        fP = make_float3(Px, Py, Pz);
+        fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
+        // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
+        const float ts = c.sigma * c.rho0 * iri;
+        fT = make_float3(ts * Tx, ts * Ty, ts * Tz);
+        if (c.has_adhesion) {
+            // Adhesion toward the walls (fluidphysics.cu:330-350), FP64.
+            const double z = -fmin(box_sdf(c, pi.x, pi.y, pi.z), 0.0) * c.quarter_inv_psize;
+            if (z < 1.0) {
+                double nbv[3];
+                box_normal(c, pi.x, pi.y, pi.z, nbv);
+                const double oz = 1.0 - z;
+                const double s = c.alpha_inv_rho0 * (double)rho_i / (oz * oz * oz);
+                fA = make_float3((float)(s * nbv[0]), (float)(s * nbv[1]), (float)(s * nbv[2]));
             }
-        } else if (kOwnNormal) {
-            N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        if (Fdbg) {
-            Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
-            Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
-            Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
-            Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
-        }
-        // Leapfrog (fluidintegrator.cu:36-69), per particle in FP64 on the FP32
-        // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
-        // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
-        // the boundary projection and reflection.
-        const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
-        const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
-        const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
-        double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
-        double s2 = ax * ax + ay * ay + az * az;
-        if (s2 > c.alim2) {
-            const double k = c.alim / sqrt(s2);
-            ax *= k;
-            ay *= k;
-            az *= k;
-        }
-        const float4 li = Ls[i];
-        double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
-        double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
-        double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
-        double vx = (double)vi.x + 0.5 * ((double)ai.x + ax) * c.dt;
-        double vy = (double)vi.y + 0.5 * ((double)ai.y + ay) * c.dt;
-        double vz = (double)vi.z + 0.5 * ((double)ai.z + az) * c.dt;
-        s2 = vx * vx + vy * vy + vz * vz;
-        if (s2 > c.vlim2) {
-            const double k = c.vlim / sqrt(s2);
-            vx *= k;
-            vy *= k;
-            vz *= k;
-        }
-        // boundaryCheckSDF (fluidintegrator.cu:11-26).
-        const double dd = box_sdf(c, x, y, z);
-        if (dd > -c.psize) {
-            double nbv[3];
-            box_normal(c, x, y, z, nbv);
-            const double push = dd + c.psize;
-            const double vn = c.vdamp1 * (nbv[0] * vx + nbv[1] * vy + nbv[2] * vz);
-            x -= push * nbv[0];
-            y -= push * nbv[1];
-            z -= push * nbv[2];
-            vx -= vn * nbv[0];
-            vy -= vn * nbv[1];
-            vz -= vn * nbv[2];
-        }
-        const float hx = (float)x, hy = (float)y, hz = (float)z;
-        Pout[i] = make_float4(hx, hy, hz, pi.w);
-        Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
-        Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
-        Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
-        X[i] = make_float4(hx, hy, hz, di.x);
-        // Move of the FP32 positions the pair predicates see (the differences
-        // are exact in FP64).  Every writer stores the same value; NaN counts
-        // as too far.
-        const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
-        if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) *far_move = 1u;
+    } else if (own_normal) {
+        N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    if (Fdbg) {
+        Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
+        Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
+        Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
+        Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
+    }
+    // Leapfrog (fluidintegrator.cu:36-69), per particle in FP64 on the FP32
+    // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
+    // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
+    // the boundary projection and reflection.
+    const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
+    const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
+    const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
+    double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
+    double s2 = ax * ax + ay * ay + az * az;
+    if (s2 > c.alim2) {
+        const double k = c.alim / sqrt(s2);
+        ax *= k;
+        ay *= k;
+        az *= k;
+    }
+    const float4 li = Ls[i];
+    double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
+    double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
+    double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
+    double vx = (double)vi.x + 0.5 * ((double)ai.x + ax) * c.dt;
+    double vy = (double)vi.y + 0.5 * ((double)ai.y + ay) * c.dt;
+    double vz = (double)vi.z + 0.5 * ((double)ai.z + az) * c.dt;
+    s2 = vx * vx + vy * vy + vz * vz;
+    if (s2 > c.vlim2) {
+        const double k = c.vlim / sqrt(s2);
+        vx *= k;
+        vy *= k;
+        vz *= k;
+    }
+    // boundaryCheckSDF (fluidintegrator.cu:11-26).
+    const double dd = box_sdf(c, x, y, z);
+    if (dd > -c.psize) {
+        double nbv[3];
+        box_normal(c, x, y, z, nbv);
+        const double push = dd + c.psize;
+        const double vn = c.vdamp1 * (nbv[0] * vx + nbv[1] * vy + nbv[2] * vz);
+        x -= push * nbv[0];
+        y -= push * nbv[1];
+        z -= push * nbv[2];
+        vx -= vn * nbv[0];
+        vy -= vn * nbv[1];
+        vz -= vn * nbv[2];
+    }
+    const float hx = (float)x, hy = (float)y, hz = (float)z;
+    Pout[i] = make_float4(hx, hy, hz, pi.w);
+    Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
+    Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
+    Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
+    X[i] = make_float4(hx, hy, hz, di.x);
+    // Move of the FP32 positions the pair predicates see (the differences
+    // are exact in FP64).  Every writer stores the same value; NaN counts
+    // as too far.
+    const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
+    if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) far_move[c.scene] = 1u;
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
@@ -533,102 +587,132 @@ __global__ __launch_bounds__(kBlock) void k_forces(Consts c, const float4 *__res
 // a pair within h after the move was within h + 2 max|dx| before it, so the
 // masks are complete while no particle moved farther than move_limit
 // (h + 2 move_limit <= h_s with FP32 margins); k_forces flags a farther move
-// in far_move, and then every row is scanned.
-__global__ __launch_bounds__(kBlock) void k_xsph(Consts c, const float4 *__restrict__ Ps,
-                                                 const float4 *__restrict__ X, const float4 *__restrict__ Vh,
-                                                 const uint32_t *__restrict__ start, RowMasks MS,
-                                                 const uint32_t *__restrict__ far_move, uint32_t n,
-                                                 float4 *__restrict__ Vout) {
-    const uint32_t i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= n) return;
+// in the scene's far_move flag, and then every row is scanned.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+                                             uint32_t nslots, const float4 *__restrict__ Ps,
+                                             const float4 *__restrict__ X, const float4 *__restrict__ Vh,
+                                             const uint32_t *__restrict__ start, RowMasks MS,
+                                             const uint32_t *__restrict__ far_move, float4 *__restrict__ Vout) {
+    uint32_t slot0;
+    const Consts *cp = block_scene<BS>(cs, chunk, slot0);
+    if (!cp) return;
+    const Consts &c = *cp;
+    const uint32_t i = slot0 + threadIdx.x;
+    if (i >= c.first + c.count) return;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
     const float4 vi = Vh[i];
-    if (i >= start[c.ncells]) {
+    if (i >= st[c.ncells] + shift) {
         Vout[i] = vi;
         return;
     }
-    const bool masks_ok = c.use_masks && *far_move == 0u;
+    const bool masks_ok = c.use_masks && far_move[c.scene] == 0u;
     const float4 xi = X[i];
     const float rho_i = xi.w;
     uint32_t b[9], e[9];
-    rows9(c, start, Ps[i], b, e);
+    rows9(c, st, shift, Ps[i], b, e);
     float sx = 0.f, sy = 0.f, sz = 0.f;
     // Both loads of a candidate are issued together; a candidate outside h
     // (or with rho_i + rho_j <= FLT_MIN) adds exact zeros.
     struct Cand {
         float4 x, v;
     };
-    walk_masks(b, e, MS, masks_ok, n, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
-               [&](const Cand &cj, uint32_t) {
-        const float4 q = cj.x;
-        const float4 vj = cj.v;
-        const float rr = dist2(xi, q);
-        const float sr = rho_i + q.w;
-        const bool in = rr <= c.h2 && sr > c.eps;
-        const float t = in ? c.h2 - rr : 0.0f;
-        const float w = in ? t * t * t * frcp(sr) : 0.0f;
-        sx += w * (vj.x - vi.x);
-        sy += w * (vj.y - vi.y);
-        sz += w * (vj.z - vi.z);
-    });
+    walk_masks(
+        b, e, MS, masks_ok, nslots, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
+        [&](const Cand &cj, uint32_t) {
+            const float4 q = cj.x;
+            const float4 vj = cj.v;
+            const float rr = dist2(xi, q);
+            const float sr = rho_i + q.w;
+            const bool in = rr <= c.h2 && sr > c.eps;
+            const float t = in ? c.h2 - rr : 0.0f;
+            const float w = in ? t * t * t * frcp(sr) : 0.0f;
+            sx += w * (vj.x - vi.x);
+            sy += w * (vj.y - vi.y);
+            sz += w * (vj.z - vi.z);
+        });
     Vout[i] = make_float4(vi.x + c.xsph_k * sx, vi.y + c.xsph_k * sy, vi.z + c.xsph_k * sz, 0.0f);
 }
 
 // ---------------------------------------------------------------------------
-// Data movement and inspection helpers.
+// Per-scene data movement and inspection helpers (not on the stepping path).
+// Slots [first, first + count) of one scene; ids are scene-local.
 
-__global__ __launch_bounds__(kBlock) void k_upload(const float *__restrict__ stage, uint32_t first, uint32_t count,
-                                                   float4 *__restrict__ P, float4 *__restrict__ V,
+__global__ __launch_bounds__(kBlock) void k_upload(const float *__restrict__ stage, uint32_t first, uint32_t id0,
+                                                   uint32_t count, float4 *__restrict__ P, float4 *__restrict__ V,
                                                    float4 *__restrict__ A, float4 *__restrict__ L) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
     if (k >= count) return;
     const float *s = stage + 9 * (size_t)k;
-    const uint32_t id = first + k;
-    P[id] = make_float4(s[0], s[1], s[2], __uint_as_float(id));
-    V[id] = make_float4(s[3], s[4], s[5], 0.0f);
-    A[id] = make_float4(s[6], s[7], s[8], 0.0f);
-    L[id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t id = id0 + k, slot = first + id;
+    P[slot] = make_float4(s[0], s[1], s[2], __uint_as_float(id));
+    V[slot] = make_float4(s[3], s[4], s[5], 0.0f);
+    A[slot] = make_float4(s[6], s[7], s[8], 0.0f);
+    L[slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(kBlock) void k_init_state(uint32_t n, float4 *__restrict__ P, float4 *__restrict__ V,
-                                                       float4 *__restrict__ A, float4 *__restrict__ L,
-                                                       float4 *__restrict__ D) {
+__global__ __launch_bounds__(kBlock) void k_init_state(uint32_t first, uint32_t count, float4 *__restrict__ P,
+                                                       float4 *__restrict__ V, float4 *__restrict__ A,
+                                                       float4 *__restrict__ L, float4 *__restrict__ D) {
     const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= n) return;
-    P[k] = make_float4(0.f, 0.f, 0.f, __uint_as_float(k));
-    V[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    A[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    L[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    D[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k >= count) return;
+    const uint32_t s = first + k;
+    P[s] = make_float4(0.f, 0.f, 0.f, __uint_as_float(k));
+    V[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    A[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    L[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    D[s] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-// Moves slot s to slot id(s): state back to particle-id order.
-__global__ __launch_bounds__(kBlock) void k_to_id_order(const float4 *__restrict__ P, const float4 *__restrict__ V,
-                                                        const float4 *__restrict__ A, const float4 *__restrict__ L,
-                                                        const float4 *__restrict__ D, uint32_t n,
+// Moves slot s to slot first + id(s): the scene's state back to id order.
+__global__ __launch_bounds__(kBlock) void k_to_id_order(uint32_t first, uint32_t count, const float4 *__restrict__ P,
+                                                        const float4 *__restrict__ V, const float4 *__restrict__ A,
+                                                        const float4 *__restrict__ L, const float4 *__restrict__ D,
                                                         float4 *__restrict__ P2, float4 *__restrict__ V2,
                                                         float4 *__restrict__ A2, float4 *__restrict__ L2,
                                                         float4 *__restrict__ D2) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t s = first + k;
     const float4 p = P[s];
-    const uint32_t id = __float_as_uint(p.w);
-    P2[id] = p;
-    V2[id] = V[s];
-    A2[id] = A[s];
-    L2[id] = L[s];
-    D2[id] = D[s];
+    const uint32_t d = first + __float_as_uint(p.w);
+    P2[d] = p;
+    V2[d] = V[s];
+    A2[d] = A[s];
+    L2[d] = L[s];
+    D2[d] = D[s];
+}
+
+// Copies `count` slots of five arrays from slot `from` of (P,V,A,L,D) to slot
+// `to` of (P2,V2,A2,L2,D2): scene relocation and per-scene reordering.  The
+// source and destination arrays are distinct.
+__global__ __launch_bounds__(kBlock) void k_move(uint32_t from, uint32_t to, uint32_t count,
+                                                 const float4 *__restrict__ P, const float4 *__restrict__ V,
+                                                 const float4 *__restrict__ A, const float4 *__restrict__ L,
+                                                 const float4 *__restrict__ D, float4 *__restrict__ P2,
+                                                 float4 *__restrict__ V2, float4 *__restrict__ A2,
+                                                 float4 *__restrict__ L2, float4 *__restrict__ D2) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    P2[to + k] = P[from + k];
+    V2[to + k] = V[from + k];
+    A2[to + k] = A[from + k];
+    L2[to + k] = L[from + k];
+    D2[to + k] = D[from + k];
 }
 
 // AoS staging for downloads: 12 floats per particle (pos, colour, vel, acc).
 __global__ __launch_bounds__(kBlock) void k_export(Consts c, const float4 *__restrict__ P,
                                                    const float4 *__restrict__ V, const float4 *__restrict__ A,
-                                                   const float4 *__restrict__ D, uint32_t n, int by_id,
-                                                   float *__restrict__ out, uint32_t *__restrict__ ids) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= n) return;
+                                                   const float4 *__restrict__ D, int by_id, float *__restrict__ out,
+                                                   uint32_t *__restrict__ ids) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.count) return;
+    const uint32_t s = c.first + k;
     const float4 p = P[s], v = V[s], a = A[s];
     const uint32_t id = __float_as_uint(p.w);
-    const uint32_t dst = by_id ? id : s;
+    const uint32_t dst = by_id ? id : k;
     const float3 col = density_colour(c, D[s].x);
     float *o = out + 12 * (size_t)dst;
     o[0] = p.x; o[1] = p.y; o[2] = p.z;
@@ -638,34 +722,41 @@ __global__ __launch_bounds__(kBlock) void k_export(Consts c, const float4 *__res
     ids[dst] = id;
 }
 
-// Neighbour sets of the density pass (debug): count, then fill.
+// Neighbour sets of the density pass (debug): count, then fill; indexed by
+// the scene-local slot.
 __global__ __launch_bounds__(kBlock) void k_nb_count(Consts c, const float4 *__restrict__ Ps,
-                                                     const uint32_t *__restrict__ start, uint32_t n,
-                                                     uint32_t *__restrict__ cnt) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    uint32_t k = 0;
-    if (i < start[c.ncells]) {
+                                                     const uint32_t *__restrict__ start, uint32_t *__restrict__ cnt) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.count) return;
+    const uint32_t i = c.first + k;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
+    uint32_t num = 0;
+    if (i < st[c.ncells] + shift) {
         const float4 pi = Ps[i];
         int cx, cy, cz;
         cell_xyz(c, pi, cx, cy, cz);
-        for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
-            if (dist2(pi, Ps[j]) <= c.h2 && j != i) ++k;
+        for_stencil(c, st, shift, cx, cy, cz, [&](uint32_t j) {
+            if (dist2(pi, Ps[j]) <= c.h2 && j != i) ++num;
         });
     }
-    cnt[i] = k;
+    cnt[k] = num;
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_fill(Consts c, const float4 *__restrict__ Ps,
-                                                    const uint32_t *__restrict__ start, uint32_t n,
+                                                    const uint32_t *__restrict__ start,
                                                     const uint64_t *__restrict__ off, uint32_t *__restrict__ list) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n || i >= start[c.ncells]) return;
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.count) return;
+    const uint32_t i = c.first + k;
+    const uint32_t *st = start + c.cell_base;
+    const uint32_t shift = c.first - st[0];
+    if (i >= st[c.ncells] + shift) return;
     const float4 pi = Ps[i];
     int cx, cy, cz;
     cell_xyz(c, pi, cx, cy, cz);
-    uint64_t o = off[i];
-    for_stencil(c, start, cx, cy, cz, [&](uint32_t j) {
+    uint64_t o = off[k];
+    for_stencil(c, st, shift, cx, cy, cz, [&](uint32_t j) {
         const float4 q = Ps[j];
         if (dist2(pi, q) <= c.h2 && j != i) list[o++] = __float_as_uint(q.w);
     });
@@ -679,105 +770,141 @@ __global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src,
 }
 
 // ---------------------------------------------------------------------------
-// Host launchers.  Every launch goes through hipExtLaunchKernelGGL so that
-// optional start/stop events are bound to the dispatch packet itself.
+// Host launchers.  A timed launch goes through hipExtLaunchKernelGGL, so that
+// its start/stop events are bound to the dispatch packet itself; an untimed
+// one through hipLaunchKernelGGL (also under stream capture).
 
-static inline unsigned blocks_for(size_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+static inline unsigned blocks_for(size_t n, unsigned bs = kBlock) { return (unsigned)((n + bs - 1) / bs); }
 
-#define LF_EXT(L, kern, nblocks, ev0, ev1, ...) \
-    hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(kBlock), 0, (L).st, (ev0), (ev1), 0, __VA_ARGS__)
+#define LF_GO(L, kern, nblocks, bs, ev0, ev1, ...)                                                        \
+    do {                                                                                                  \
+        if ((ev0) || (ev1))                                                                               \
+            hipExtLaunchKernelGGL(kern, dim3(nblocks), dim3(bs), 0, (L).st, (ev0), (ev1), 0, __VA_ARGS__); \
+        else                                                                                              \
+            hipLaunchKernelGGL(kern, dim3(nblocks), dim3(bs), 0, (L).st, __VA_ARGS__);                    \
+    } while (0)
 
-void launch_hash(const Launch &L, const Consts &c, const float4 *P, uint32_t n, uint32_t *keys, uint32_t *ranks,
+// Runs body.template operator()<BS>() for the block size bs (128, 256, 512).
+template <class F>
+static inline void with_bs(int bs, F &&body) {
+    switch (bs) {
+        case 128: body.template operator()<128>(); break;
+        case 512: body.template operator()<512>(); break;
+        default: body.template operator()<256>(); break;
+    }
+}
+
+bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
+
+void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts) {
-    if (n) LF_EXT(L, k_hash, blocks_for(n), L.a, L.b, c, P, n, keys, ranks, counts);
+    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
     const uint32_t nb = scan_tiles(m);
-    LF_EXT(L, k_scan_sum, nb, L.a, nullptr, (const uint32_t *)counts, m, bsum);
-    LF_EXT(L, k_scan_top, 1, nullptr, nullptr, bsum, nb, start, m);
-    LF_EXT(L, k_scan_down, nb, nullptr, L.b, counts, m, (const uint32_t *)bsum, start);
+    LF_GO(L, k_scan_sum, nb, kBlock, L.a, nullptr, (const uint32_t *)counts, m, bsum);
+    LF_GO(L, k_scan_top, 1, kBlock, nullptr, nullptr, bsum, nb, start, m);
+    LF_GO(L, k_scan_down, nb, kBlock, nullptr, L.b, counts, m, (const uint32_t *)bsum, start);
 }
 
-void launch_scatter(const Launch &L, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
-                    const uint32_t *start, uint32_t n, uint32_t *slot_src, uint32_t *slot_id) {
-    if (n) LF_EXT(L, k_scatter, blocks_for(n), L.a, L.b, P, keys, ranks, start, n, slot_src, slot_id);
+void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
+                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id) {
+    LF_GO(L, k_scatter, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, start, slot_src,
+          slot_id);
 }
 
-void launch_gather(const Launch &L, const uint32_t *slot_src, const uint32_t *slot_id, const uint32_t *keys,
-                   const uint32_t *start, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
-                   uint32_t n, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err) {
-    if (n)
-        LF_EXT(L, k_gather, blocks_for(n), L.a, L.b, slot_src, slot_id, keys, start, P, V, A, Lw, n, Ps, Vs, As, Ls,
-               err);
+void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
+                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err) {
+    LF_GO(L, k_gather, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
+          A, Lw, Ps, Vs, As, Ls, err);
 }
 
-void launch_density(const Launch &L, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n, float4 *D,
+void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const uint32_t *start, float4 *D,
                     float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move) {
-    if (n) LF_EXT(L, k_density, blocks_for(n), L.a, L.b, c, Ps, start, n, D, Q, Pr, M, MS, far_move);
+    with_bs(bs, [&]<int BS>() {
+        LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, start, D, Q, Pr, M, MS,
+              far_move);
+    });
 }
 
-void launch_normals(const Launch &L, const Consts &c, const float4 *Q, const uint32_t *start, RowMasks M,
-                    uint32_t n, float4 *N) {
-    if (n) LF_EXT(L, k_normals, blocks_for(n), L.a, L.b, c, Q, start, M, n, N);
+void launch_normals(const Launch &L, const PoolView &pv, int bs, const float4 *Q, const uint32_t *start, RowMasks M,
+                    float4 *N) {
+    with_bs(bs, [&]<int BS>() {
+        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, N);
+    });
 }
 
-void launch_forces(const Launch &L, const Consts &c, bool own_normal, const float4 *Ps, const float4 *Q,
-                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
-                   const uint32_t *start, RowMasks M, uint32_t n, float4 *Pout, float4 *Vout, float4 *Aout,
-                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move) {
-    if (!n) return;
-    if (own_normal)
-        LF_EXT(L, k_forces<true>, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout,
-               Aout, Lout, X, Fdbg, far_move);
-    else
-        LF_EXT(L, k_forces<false>, blocks_for(n), L.a, L.b, c, Ps, Q, Pr, Vs, As, Ls, D, N, start, M, n, Pout, Vout,
-               Aout, Lout, X, Fdbg, far_move);
+void launch_forces(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *Q, const float *Pr,
+                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
+                   const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
+                   float4 *X, float4 *Fdbg, uint32_t *far_move) {
+    with_bs(bs, [&]<int BS>() {
+        LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
+              start, M, Pout, Vout, Aout, Lout, X, Fdbg, far_move);
+    });
 }
 
-void launch_xsph(const Launch &L, const Consts &c, const float4 *Ps, const float4 *X, const float4 *Vh,
-                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, uint32_t n, float4 *Vout) {
-    if (n) LF_EXT(L, k_xsph, blocks_for(n), L.a, L.b, c, Ps, X, Vh, start, MS, far_move, n, Vout);
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *X, const float4 *Vh,
+                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout) {
+    with_bs(bs, [&]<int BS>() {
+        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, start, MS, far_move,
+              Vout);
+    });
 }
 
-void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t count, float4 *P, float4 *V,
-                   float4 *A, float4 *Lw) {
+void launch_upload(hipStream_t st, const float *stage, uint32_t first, uint32_t id0, uint32_t count, float4 *P,
+                   float4 *V, float4 *A, float4 *Lw) {
     if (count)
-        hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, count, P, V, A, Lw);
+        hipLaunchKernelGGL(k_upload, dim3(blocks_for(count)), dim3(kBlock), 0, st, stage, first, id0, count, P, V, A,
+                           Lw);
 }
 
-void launch_init_state(hipStream_t st, uint32_t n, float4 *P, float4 *V, float4 *A, float4 *Lw, float4 *D) {
-    if (n) hipLaunchKernelGGL(k_init_state, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, P, V, A, Lw, D);
+void launch_init_state(hipStream_t st, uint32_t first, uint32_t count, float4 *P, float4 *V, float4 *A, float4 *Lw,
+                       float4 *D) {
+    if (count)
+        hipLaunchKernelGGL(k_init_state, dim3(blocks_for(count)), dim3(kBlock), 0, st, first, count, P, V, A, Lw, D);
 }
 
-void launch_to_id_order(hipStream_t st, const float4 *P, const float4 *V, const float4 *A, const float4 *Lw,
-                        const float4 *D, uint32_t n, float4 *P2, float4 *V2, float4 *A2, float4 *L2, float4 *D2) {
-    if (n)
-        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(n)), dim3(kBlock), 0, st, P, V, A, Lw, D, n, P2, V2, A2, L2,
-                           D2);
+void launch_to_id_order(hipStream_t st, uint32_t first, uint32_t count, const float4 *P, const float4 *V,
+                        const float4 *A, const float4 *Lw, const float4 *D, float4 *P2, float4 *V2, float4 *A2,
+                        float4 *L2, float4 *D2) {
+    if (count)
+        hipLaunchKernelGGL(k_to_id_order, dim3(blocks_for(count)), dim3(kBlock), 0, st, first, count, P, V, A, Lw, D,
+                           P2, V2, A2, L2, D2);
+}
+
+void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, const float4 *P, const float4 *V,
+                 const float4 *A, const float4 *Lw, const float4 *D, float4 *P2, float4 *V2, float4 *A2, float4 *L2,
+                 float4 *D2) {
+    if (count)
+        hipLaunchKernelGGL(k_move, dim3(blocks_for(count)), dim3(kBlock), 0, st, from, to, count, P, V, A, Lw, D, P2,
+                           V2, A2, L2, D2);
 }
 
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *D, uint32_t n, int by_id, float *out, uint32_t *ids) {
-    if (n) hipLaunchKernelGGL(k_export, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, P, V, A, D, n, by_id, out, ids);
+                   const float4 *D, bool by_id, float *out, uint32_t *ids) {
+    if (c.count)
+        hipLaunchKernelGGL(k_export, dim3(blocks_for(c.count)), dim3(kBlock), 0, st, c, P, V, A, D, by_id ? 1 : 0,
+                           out, ids);
 }
 
-void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                     uint32_t *cnt) {
-    if (n) hipLaunchKernelGGL(k_nb_count, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, cnt);
+void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t *cnt) {
+    if (c.count) hipLaunchKernelGGL(k_nb_count, dim3(blocks_for(c.count)), dim3(kBlock), 0, st, c, Ps, start, cnt);
 }
 
-void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t n,
-                    const uint64_t *off, uint32_t *list) {
-    if (n) hipLaunchKernelGGL(k_nb_fill, dim3(blocks_for(n)), dim3(kBlock), 0, st, c, Ps, start, n, off, list);
+void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, const uint64_t *off,
+                    uint32_t *list) {
+    if (c.count)
+        hipLaunchKernelGGL(k_nb_fill, dim3(blocks_for(c.count)), dim3(kBlock), 0, st, c, Ps, start, off, list);
 }
 
 void launch_copy(const Launch &L, const float4 *src, float4 *dst, size_t n4) {
-    const size_t want = (n4 + kBlock - 1) / kBlock;
-    const unsigned nb = (unsigned)(want < 65536 ? (want ? want : 1) : 65536);
-    LF_EXT(L, k_copy, nb, L.a, L.b, src, dst, n4);
+    const unsigned nb = (unsigned)std::min<size_t>(std::max<size_t>(blocks_for(n4), 1), 65536);
+    LF_GO(L, k_copy, nb, kBlock, L.a, L.b, src, dst, n4);
 }
 
 }  // namespace lf

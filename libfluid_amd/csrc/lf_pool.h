@@ -1,0 +1,150 @@
+// lf_pool.h -- host side of a pool of independent scenes that step together
+// (internal to libfluid_hip.so).
+//
+// A pool owns one set of device arrays sized for `slot_cap` particle slots
+// and `bucket_cap` cell buckets, plus a table of per-scene constants.  Scenes
+// are admitted into free slot and bucket ranges (first fit, compacting when
+// fragmented) and removed at any time; every substep is one launch per stage
+// over the whole pool, either eager or replayed from a captured hipGraph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "lf_abi.h"
+#include "lf_device.h"
+#include "lf_kernels.h"
+
+namespace lf {
+
+// Workgroup sizes of the four neighbour passes (128, 256 or 512 each).
+struct Tuning {
+    int density = 256, normals = 256, forces = 256, xsph = 256;
+};
+
+struct SceneRec {
+    bool live = false;
+    bool active = false;       // stepped by step()
+    lf_params params{};
+    uint32_t n = 0;            // particles
+    uint32_t nchunks = 0;      // slots [first, first + nchunks * kChunk)
+    uint32_t first = 0;
+    uint32_t ncells = 0;       // buckets [cell_base, cell_base + ncells + 1)
+    uint32_t cell_base = 0;
+    float dt = 0.0f;           // substep length
+    bool identity = true;      // state arrays in particle-id order
+    uint64_t steps = 0;
+};
+
+class Pool {
+  public:
+    Pool() = default;
+    Pool(const Pool &) = delete;
+    Pool &operator=(const Pool &) = delete;
+    ~Pool();
+
+    int init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap);
+
+    // Scenes.  add_scene returns LF_ERR_FULL when the pool cannot hold it.
+    int add_scene(const lf_params &p, uint32_t n, float dt, uint32_t *id);
+    int remove_scene(uint32_t id);
+    int set_params(uint32_t id, const lf_params &p);
+    int set_dt(uint32_t id, float dt);
+    int set_active(uint32_t id, bool on);
+    const SceneRec *scene(uint32_t id) const;
+    uint32_t scene_cap() const { return (uint32_t)scenes_.size(); }
+    uint32_t slot_cap() const { return nslots_; }
+    uint32_t bucket_cap() const { return nbuckets_; }
+    uint32_t live_scenes() const;
+    void usage(uint32_t *slots_used, uint32_t *buckets_used) const;
+
+    // Data in / out of one scene (synchronous).
+    int upload(uint32_t id, uint32_t first, uint32_t count, const float *pos, const float *vel, const float *acc);
+    int download(uint32_t id, int order, float *pos, float *col, float *vel, float *acc, uint32_t *ids);
+
+    // Enqueues `substeps` substeps of every active scene and returns.
+    int step(uint32_t substeps);
+    int sync();
+
+    // Inspection of the last substep, tuning, timing.
+    int set_debug(int flags);
+    int get_field(uint32_t id, int field, void *dst, size_t bytes);
+    int neighbours(uint32_t id, uint32_t *counts, uint32_t *list, uint64_t cap, uint64_t *total);
+    int profile_enable(uint32_t mask);
+    int profile_read(uint64_t *launches, double *ms);
+    int copy_probe(size_t bytes, int repeats, double *ms);
+    int set_tuning(const Tuning &t);
+    const Tuning &tuning() const { return tune_; }
+    // Times every block size of the neighbour passes on the pool's current
+    // scenes (reps substeps each, state restored afterwards) and keeps the
+    // fastest per pass.  ms[k][b]: average time of pass k (density, normals,
+    // forces, xsph) at block size 128 << b; may be null.
+    int autotune(uint32_t reps, double ms[4][3]);
+    void set_graphs(bool on);
+
+  private:
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    template <class T>
+    int dalloc(T **p, size_t count);
+    void release();
+    Consts make_scene_consts(uint32_t id) const;
+    int push_tables();
+    int find_slots(uint32_t nchunks, uint32_t *first) const;
+    int find_buckets(uint32_t nb, uint32_t *base) const;
+    int compact();
+    int to_id_order(uint32_t id);
+    bool any_separate_normals() const;
+    void enqueue_sort();
+    void enqueue_substep();
+    int run_substeps(uint32_t substeps);
+    hipEvent_t take_event();
+    template <class Fn>
+    void timed(int kind, Fn &&launch);
+    void drop_graph();
+    PoolView view() const { return PoolView{d_cs_, d_chunk_, nslots_}; }
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    uint32_t nslots_ = 0, nbuckets_ = 0, ntiles_ = 0;
+    int debug_ = 0;
+    Tuning tune_{};
+    bool use_graphs_ = true;
+
+    // device arrays: state (P0,V0,A0,L0) and the substep's working arrays
+    float4 *P0_ = nullptr, *P1_ = nullptr, *V0_ = nullptr, *V1_ = nullptr, *V2_ = nullptr;
+    float4 *A0_ = nullptr, *A1_ = nullptr, *L0_ = nullptr, *L1_ = nullptr;
+    float4 *D_ = nullptr, *N_ = nullptr, *F_ = nullptr, *Q_ = nullptr, *X_ = nullptr;
+    float *Pr_ = nullptr;
+    RowMasks M_{nullptr, nullptr}, MS_{nullptr, nullptr};
+    uint32_t *keys_ = nullptr, *ranks_ = nullptr, *slot_src_ = nullptr, *slot_id_ = nullptr;
+    uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr, *err_ = nullptr;
+    uint32_t *far_move_ = nullptr;  // per scene
+    Consts *d_cs_ = nullptr;
+    uint32_t *d_chunk_ = nullptr;
+
+    // host tables
+    std::vector<SceneRec> scenes_;
+    std::vector<Consts> h_cs_;
+    std::vector<uint32_t> h_chunk_;
+    bool tables_dirty_ = true;
+
+    // captured substep
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    bool graph_normals_ = false;  // the captured substep has the normals pass
+
+    // profiling
+    uint32_t prof_mask_ = 0;
+    std::vector<ProfRec> prof_live_;
+    std::vector<hipEvent_t> ev_pool_;
+};
+
+// Kernel constants of a scene with parameters p, substep dt and the given
+// debug flags (placement fields left zero).
+Consts make_consts(const lf_params &p, float dt, int debug_flags);
+
+}  // namespace lf

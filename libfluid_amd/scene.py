@@ -158,6 +158,154 @@ class Scene:
         check(lib.lf_copy_probe(self._h, int(nbytes), int(repeats), C.byref(ms)), "lf_copy_probe")
         return ms.value
 
+    # -- launch shapes ------------------------------------------------------
+    def set_tuning(self, block_sizes) -> None:
+        """Workgroup sizes of the density, normals, forces and XSPH passes."""
+        bs = (C.c_int * 4)(*[int(b) for b in block_sizes])
+        check(lib.lf_set_tuning(self._h, bs), "lf_set_tuning")
+
+    def tuning(self) -> tuple:
+        bs = (C.c_int * 4)()
+        check(lib.lf_get_tuning(self._h, bs), "lf_get_tuning")
+        return tuple(bs)
+
+    def autotune(self, reps: int = 5) -> np.ndarray:
+        """Keeps the fastest block size per pass; returns the (4, 3) ms table
+        (passes density/normals/forces/xsph x block sizes 128/256/512)."""
+        ms = (C.c_double * 12)()
+        check(lib.lf_autotune(self._h, int(reps), ms), "lf_autotune")
+        return np.array(ms[:], dtype=np.float64).reshape(4, 3)
+
+    def set_graphs(self, on: bool) -> None:
+        check(lib.lf_set_graphs(self._h, 1 if on else 0), "lf_set_graphs")
+
+
+class Batch:
+    """Many independent scenes stepped together (lf_batch_*)."""
+
+    def __init__(self, max_particles: int, max_cells: int, max_scenes: int):
+        self._h = C.c_void_p()
+        check(lib.lf_batch_create(int(max_particles), int(max_cells), int(max_scenes), C.byref(self._h)),
+              "lf_batch_create")
+        self._n = {}
+
+    @staticmethod
+    def footprint(max_particles: int, max_cells: int, max_scenes: int) -> int:
+        b = C.c_uint64(0)
+        check(lib.lf_batch_footprint(int(max_particles), int(max_cells), int(max_scenes), C.byref(b)),
+              "lf_batch_footprint")
+        return b.value
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            lib.lf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, params: LfParams, num_particles: int, dt: float) -> int:
+        sid = C.c_uint32(0)
+        check(lib.lf_batch_add(self._h, C.byref(params), int(num_particles), C.c_float(dt), C.byref(sid)),
+              "lf_batch_add")
+        self._n[sid.value] = int(num_particles)
+        return sid.value
+
+    def try_add(self, params: LfParams, num_particles: int, dt: float):
+        """Like add, but returns None when the batch has no room."""
+        sid = C.c_uint32(0)
+        rc = lib.lf_batch_add(self._h, C.byref(params), int(num_particles), C.c_float(dt), C.byref(sid))
+        if rc == L.LF_ERR_FULL:
+            return None
+        check(rc, "lf_batch_add")
+        self._n[sid.value] = int(num_particles)
+        return sid.value
+
+    def remove(self, scene: int) -> None:
+        check(lib.lf_batch_remove(self._h, int(scene)), "lf_batch_remove")
+        self._n.pop(int(scene), None)
+
+    def set_params(self, scene: int, params: LfParams) -> None:
+        check(lib.lf_batch_set_params(self._h, int(scene), C.byref(params)), "lf_batch_set_params")
+
+    def set_dt(self, scene: int, dt: float) -> None:
+        check(lib.lf_batch_set_dt(self._h, int(scene), C.c_float(dt)), "lf_batch_set_dt")
+
+    def set_active(self, scene: int, active: bool) -> None:
+        check(lib.lf_batch_set_active(self._h, int(scene), 1 if active else 0), "lf_batch_set_active")
+
+    def upload(self, scene: int, pos: np.ndarray, vel: np.ndarray | None = None, acc: np.ndarray | None = None,
+               first: int = 0) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        acc = None if acc is None else np.ascontiguousarray(acc, dtype=np.float32).reshape(-1, 3)
+        check(lib.lf_batch_upload(self._h, int(scene), int(first), pos.shape[0], fptr(pos), fptr(vel), fptr(acc)),
+              "lf_batch_upload")
+
+    def download(self, scene: int, order: int = L.LF_ORDER_ID) -> dict:
+        n = self._n[int(scene)]
+        out = {k: np.empty((n, 3), np.float32) for k in ("pos", "colour", "vel", "acc")}
+        ids = np.empty(n, np.uint32)
+        check(lib.lf_batch_download(self._h, int(scene), int(order), fptr(out["pos"]), fptr(out["colour"]),
+                                    fptr(out["vel"]), fptr(out["acc"]), uptr(ids)), "lf_batch_download")
+        out["ids"] = ids
+        return out
+
+    def step(self, substeps: int = 1) -> None:
+        check(lib.lf_batch_step(self._h, int(substeps)), "lf_batch_step")
+
+    def sync(self) -> None:
+        check(lib.lf_batch_sync(self._h), "lf_batch_sync")
+
+    def scene_info(self, scene: int) -> dict:
+        n, steps, act = C.c_uint32(0), C.c_uint64(0), C.c_int(0)
+        check(lib.lf_batch_scene_info(self._h, int(scene), C.byref(n), C.byref(steps), C.byref(act)),
+              "lf_batch_scene_info")
+        return {"particles": n.value, "steps": steps.value, "active": bool(act.value)}
+
+    def usage(self) -> dict:
+        v = [C.c_uint32(0) for _ in range(5)]
+        check(lib.lf_batch_usage(self._h, *[C.byref(x) for x in v]), "lf_batch_usage")
+        return dict(zip(("scenes", "slots_used", "slot_cap", "buckets_used", "bucket_cap"), [x.value for x in v]))
+
+    def set_tuning(self, block_sizes) -> None:
+        bs = (C.c_int * 4)(*[int(b) for b in block_sizes])
+        check(lib.lf_batch_set_tuning(self._h, bs), "lf_batch_set_tuning")
+
+    def tuning(self) -> tuple:
+        bs = (C.c_int * 4)()
+        check(lib.lf_batch_get_tuning(self._h, bs), "lf_batch_get_tuning")
+        return tuple(bs)
+
+    def autotune(self, reps: int = 5) -> np.ndarray:
+        ms = (C.c_double * 12)()
+        check(lib.lf_batch_autotune(self._h, int(reps), ms), "lf_batch_autotune")
+        return np.array(ms[:], dtype=np.float64).reshape(4, 3)
+
+    def set_graphs(self, on: bool) -> None:
+        check(lib.lf_batch_set_graphs(self._h, 1 if on else 0), "lf_batch_set_graphs")
+
+    def profile(self, kernels=()) -> None:
+        mask = 0
+        for k in kernels:
+            mask |= 1 << L.K_INDEX[k]
+        check(lib.lf_batch_profile_enable(self._h, mask), "lf_batch_profile_enable")
+
+    def profile_read(self) -> dict:
+        cnt = (C.c_uint64 * len(L.KERNELS))()
+        ms = (C.c_double * len(L.KERNELS))()
+        check(lib.lf_batch_profile_read(self._h, cnt, ms), "lf_batch_profile_read")
+        return {k: (int(cnt[i]), float(ms[i])) for i, k in enumerate(L.KERNELS)}
+
 
 # -- host-side generators (no device needed) ----------------------------------
 

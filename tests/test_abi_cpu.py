@@ -15,6 +15,7 @@ import pytest
 
 import libfluid_amd as lf
 from libfluid_amd import _lib as L
+from libfluid_amd import configs
 from libfluid_amd.scene import default_params, fluid_block, jittered_lattice, uniform_points, write_geo
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -83,6 +84,45 @@ def test_create_without_device_fails_cleanly():
     h = C.c_void_p()
     rc = L.lib.lf_create(C.byref(default_params(8)), 100, C.byref(h))
     assert rc < 0 and not h.value and L.lib.lf_last_error()
+
+
+def test_batch_create_without_device_fails_cleanly():
+    if lf.device_count() > 0:
+        pytest.skip("a device is visible")
+    h = C.c_void_p()
+    rc = L.lib.lf_batch_create(4096, 4096, 4, C.byref(h))
+    assert rc < 0 and not h.value and L.lib.lf_last_error()
+
+
+def test_batch_footprint():
+    """Bytes a batch allocates: 444 per slot (slots rounded up to 512), 8 per
+    bucket, a scene-table entry per scene, 4 per 512-slot chunk."""
+    b1 = lf.Batch.footprint(1, 10, 1)
+    b2 = lf.Batch.footprint(512, 10, 1)
+    assert b1 == b2  # one chunk either way
+    b3 = lf.Batch.footprint(1024, 10, 1)
+    assert b3 - b2 == 512 * 444 + 4
+    assert lf.Batch.footprint(512, 20, 1) - b2 == 80
+    per_scene = lf.Batch.footprint(512, 10, 2) - b2
+    assert 200 < per_scene < 1024
+
+
+def test_batched_c4_workload_is_deterministic():
+    """256 scenes, counts log-uniform in [2048, 65536], boxes covering their
+    blocks, unit gravity directions scaled to 9.8."""
+    a = configs.batched_c4()
+    b = configs.batched_c4()
+    assert len(a.scenes) == 256
+    ns = np.array([s.n for s in a.scenes])
+    assert ns.min() >= 2048 and ns.max() <= 65536
+    assert 3.5e6 < ns.sum() < 6.5e6  # E[n] = (65536 - 2048) / ln 32 ~ 18.3k
+    for s, t in zip(a.scenes, b.scenes):
+        assert s.n == t.n and np.array_equal(s.pos, t.pos)
+        g = np.array(s.params.gravity[:], np.float64)
+        assert abs(np.linalg.norm(g) - 9.8) < 1e-4
+        box = np.array(s.params.box[:]), np.array(s.params.grid[:]) * 0.02
+        assert np.all(s.pos.max(0) < box[0]) and np.all(box[1] >= box[0] - 1e-6)
+        assert s.pos.shape == (s.n, 3)
 
 
 def test_fluid_block_lattice():
