@@ -22,6 +22,13 @@ own boxes and gravity directions) stepped as one batch (autotuned block
 sizes, one captured hipGraph per substep) against the same scenes stepped one
 after another, each alone, for the same number of substeps; the final states
 of the two runs are compared bit for bit.
+
+served: a seeded Poisson stream of jobs (4k-256k particles, 200-2000 steps,
+own box and gravity each) through the continuous-batching server under an
+HBM budget (--serve-budget-gb; the rate gives an offered load of 0.9 of the
+measured batch throughput); aggregate particle-steps/s, p50/p99 per-step and
+per-job latency, mean occupancy, and the rate of the same jobs stepped alone
+on a sample of them.
 """
 from __future__ import annotations
 
@@ -59,6 +66,9 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, launch the plain copy kernel once over MIB MiB (calibrates byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--serve-budget-gb", type=float, default=1.0, help="device memory budget of the server")
+    ap.add_argument("--serve-jobs", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--scan", action="store_true",
@@ -125,6 +135,59 @@ def batched(args) -> dict:
         "bitwise_equal_to_sequential": bool(same),
         "tuning": list(tuning),
         "autotune_ms": np.round(tune_ms, 4).tolist(),
+    }
+
+
+def served(args, batch_rate: float) -> dict:
+    """Poisson stream through the continuous-batching server."""
+    import libfluid_amd as lf
+    from libfluid_amd import _lib as L
+    from libfluid_amd.serve import Server, drive, served_jobs
+
+    budget = int(args.serve_budget_gb * 2**30)
+    probe = served_jobs(args.serve_jobs, rate=1.0, seed=5)
+    work = float(np.mean([j.cfg.n * j.steps for j in probe]))
+    rate = 0.9 * batch_rate / work
+    jobs = served_jobs(args.serve_jobs, rate=rate, seed=5)
+    srv = Server.for_budget(budget, cells_per_particle=1.5, max_scenes=1024)
+    cap = srv.stats()["slot_cap"]
+    log(f"[bench] serve: {len(jobs)} jobs, rate {rate:.2f}/s, budget {args.serve_budget_gb} GB = {cap} slots")
+    r = drive(srv, jobs, log=log)
+    st = srv.stats()
+    srv.close()
+    value = st["particle_steps"] / r["wall_s"]
+    lat = np.array(list(r["latency_ms"].values()))
+    # The same jobs alone, on a sample: 8 jobs, 100 substeps each after 2 warm-up substeps.
+    rng = np.random.Generator(np.random.PCG64(6))
+    sample = rng.choice(len(jobs), size=min(8, len(jobs)), replace=False)
+    ps, ts = 0, 0.0
+    for k in sample:
+        c = jobs[int(k)].cfg
+        sc = lf.Scene(c.params, c.n)
+        sc.upload(c.pos, c.vel)
+        sc.run(c.dt, 2)
+        sc.sync()
+        t0 = time.perf_counter()
+        sc.run(c.dt, 100)
+        sc.sync()
+        ts += time.perf_counter() - t0
+        ps += c.n * 100
+        sc.close()
+    seq = ps / ts
+    log(f"[bench] serve: {value:.4e} particle-steps/s over {r['wall_s']:.2f} s, alone {seq:.4e}, "
+        f"occupancy {st['mean_occupancy']:.2f}, step p50/p99 {st['step_p50_ms']:.2f}/{st['step_p99_ms']:.2f} ms")
+    return {
+        "workload": f"{len(jobs)} jobs, Poisson rate {rate:.3f}/s (offered load 0.9), particles log-uniform in "
+                    f"[4096, 262144], steps uniform in [200, 2000], own box and gravity each, dt=1e-4",
+        "budget_bytes": budget, "slot_cap": cap, "jobs": len(jobs),
+        "value": value, "unit": "particle-steps/s",
+        "busy_value": st["particle_steps"] / st["busy_seconds"],
+        "wall_s": r["wall_s"], "particle_steps": st["particle_steps"],
+        "mean_occupancy": st["mean_occupancy"],
+        "step_p50_ms": st["step_p50_ms"], "step_p99_ms": st["step_p99_ms"],
+        "job_p50_ms": float(np.percentile(lat, 50)), "job_p99_ms": float(np.percentile(lat, 99)),
+        "wait_p50_ms": st["wait_p50_ms"], "wait_p99_ms": st["wait_p99_ms"],
+        "alone_value_sample": seq, "speedup_vs_alone": value / seq,
     }
 
 
@@ -287,6 +350,8 @@ def main():
         "vs_baseline_ref": "README.md:2 of the reference, 1M particles at 77 FPS on an RTX 2080 = 7.7e7 particle-steps/s",
     }
     out["batched"] = None if args.no_batch else batched(args)
+    rate_ref = out["batched"]["value"] if out["batched"] else value
+    out["served"] = None if args.no_serve else served(args, rate_ref)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)

@@ -225,6 +225,56 @@ int lf_batch_set_graphs(lf_batch *b, int on);
 int lf_batch_profile_enable(lf_batch *b, int mask);
 int lf_batch_profile_read(lf_batch *b, uint64_t *launches, double *ms);
 
+/* ---- serving: continuous batching of scene jobs -----------------------
+ * No counterpart in the reference.  A job is a scene (parameters, initial
+ * state) plus a number of substeps of length dt.  lf_server_tick admits
+ * queued jobs into free pool memory in arrival order (LF_SERVE_BACKFILL lets
+ * a later job pass one that does not fit yet; LF_SERVE_FIFO does not),
+ * advances every running job by one substep in one batched step, and retires
+ * jobs that reached their step count: their final state is kept in host
+ * memory until lf_server_job_result and their pool memory is reused at
+ * once.  Ticks do not wait for the device (at most two batched steps are
+ * queued ahead of it); a job is reported finished once its final state has
+ * reached host memory.  A job's result is bitwise what the same scene gives
+ * alone in an lf_scene.  Single-threaded: one caller drives a server. */
+typedef struct lf_server lf_server;
+enum lf_serve_policy { LF_SERVE_FIFO = 0, LF_SERVE_BACKFILL = 1 };
+enum lf_job_state { LF_JOB_QUEUED = 0, LF_JOB_RUNNING = 1, LF_JOB_DONE = 2 };
+typedef struct lf_server_stats {
+    uint64_t submitted, admitted, finished, queued, running;
+    uint64_t ticks;              /* lf_server_tick calls */
+    uint64_t step_ticks;         /* ticks that stepped at least one job */
+    uint64_t particle_steps;     /* sum over stepping ticks of the particles stepped */
+    uint32_t resident_particles; /* particles of running jobs now */
+    uint32_t slot_cap;           /* particle slots of the pool */
+    double busy_seconds;         /* wall time spent inside lf_server_tick */
+    double step_p50_ms, step_p99_ms; /* device time of one batched step (all running jobs) */
+    double job_p50_ms, job_p99_ms;   /* submit to finish, finished jobs */
+    double wait_p50_ms, wait_p99_ms; /* submit to admission, admitted jobs */
+    double mean_occupancy;       /* particles stepped / slot_cap, mean over stepping ticks */
+} lf_server_stats;
+/* Capacity as for lf_batch_create (see lf_batch_footprint for the bytes). */
+int lf_server_create(uint32_t max_particles, uint32_t max_cells, uint32_t max_scenes, int policy, lf_server **out);
+int lf_server_destroy(lf_server *s);
+/* Copies the initial state (vel may be NULL); LF_ERR_FULL if the job could
+ * never fit this server. */
+int lf_server_submit(lf_server *s, const lf_params *p, uint32_t num_particles, const float *pos_xyz,
+                     const float *vel_xyz, float dt, uint32_t steps, uint64_t *job);
+/* One serving iteration; *finished = jobs whose results arrived during it. */
+int lf_server_tick(lf_server *s, uint32_t *finished);
+/* run_ahead: batched steps a tick may queue ahead of the device (default 2;
+ * 0 makes every tick wait for its step and retirements); graphs: replay
+ * captured substeps (default on).  Results do not depend on either. */
+int lf_server_configure(lf_server *s, uint32_t run_ahead, int graphs);
+/* Moves up to cap ids of jobs finished since the last call into jobs[]. */
+int lf_server_take_finished(lf_server *s, uint64_t *jobs, uint32_t cap, uint32_t *count);
+int lf_server_job_status(const lf_server *s, uint64_t job, int *state, uint32_t *steps_done);
+/* Final state of a finished job in particle-id order (any pointer may be
+ * NULL) and its submit-to-finish latency; forgets the job. */
+int lf_server_job_result(lf_server *s, uint64_t job, float *pos_xyz, float *colour_rgb, float *vel_xyz,
+                         float *acc_xyz, double *latency_ms);
+int lf_server_get_stats(const lf_server *s, lf_server_stats *out);
+
 /* ---- host-side helpers (no device needed) ---------------------------- */
 /* createFluidBlock (fluidsystem.cu:66-127): lattice of up to n points in the
  * box [mn, mx], x outermost and z innermost, FP32 float accumulators as in

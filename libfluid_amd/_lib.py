@@ -39,6 +39,8 @@ DEBUG_SCAN = 2
 DEBUG_NORMALS_PASS = 4
 
 LF_ERR_FULL = -6
+SERVE_FIFO, SERVE_BACKFILL = 0, 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE = 0, 1, 2
 
 KERNELS = ("hash", "scan", "scatter", "gather", "density", "normals", "forces", "xsph")
 K_INDEX = {k: i for i, k in enumerate(KERNELS)}
@@ -77,6 +79,20 @@ class LfParams(C.Structure):
     @property
     def ncells(self) -> int:
         return int(self.grid[0]) * int(self.grid[1]) * int(self.grid[2])
+
+
+class LfServerStats(C.Structure):
+    """Mirror of ``lf_server_stats`` (include/lf_abi.h)."""
+
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "admitted", "finished", "queued", "running", "ticks",
+                                           "step_ticks", "particle_steps")] + [
+        ("resident_particles", C.c_uint32),
+        ("slot_cap", C.c_uint32),
+    ] + [(k, C.c_double) for k in ("busy_seconds", "step_p50_ms", "step_p99_ms", "job_p50_ms", "job_p99_ms",
+                                   "wait_p50_ms", "wait_p99_ms", "mean_occupancy")]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LfError(RuntimeError):
@@ -147,6 +163,16 @@ def _load():
         "lf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_profile_read": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_double)]),
+        "lf_server_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P(C.c_void_p)]),
+        "lf_server_destroy": (C.c_int, [C.c_void_p]),
+        "lf_server_submit": (C.c_int, [C.c_void_p, P(LfParams), C.c_uint32, f32p, f32p, C.c_float, C.c_uint32,
+                                       P(C.c_uint64)]),
+        "lf_server_tick": (C.c_int, [C.c_void_p, u32p]),
+        "lf_server_configure": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int]),
+        "lf_server_take_finished": (C.c_int, [C.c_void_p, P(C.c_uint64), C.c_uint32, u32p]),
+        "lf_server_job_status": (C.c_int, [C.c_void_p, C.c_uint64, P(C.c_int), u32p]),
+        "lf_server_job_result": (C.c_int, [C.c_void_p, C.c_uint64, f32p, f32p, f32p, f32p, P(C.c_double)]),
+        "lf_server_get_stats": (C.c_int, [C.c_void_p, P(LfServerStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -168,7 +194,8 @@ EXPORTED = (
     "lf_batch_set_params lf_batch_set_dt lf_batch_set_active lf_batch_upload lf_batch_download "
     "lf_batch_step lf_batch_sync lf_batch_scene_info lf_batch_usage lf_batch_set_tuning "
     "lf_batch_get_tuning lf_batch_autotune lf_batch_set_graphs lf_batch_profile_enable "
-    "lf_batch_profile_read"
+    "lf_batch_profile_read lf_server_create lf_server_destroy lf_server_submit lf_server_tick "
+    "lf_server_job_status lf_server_job_result lf_server_get_stats lf_server_take_finished lf_server_configure"
 ).split()
 
 

@@ -62,6 +62,13 @@ class Pool {
     // Data in / out of one scene (synchronous).
     int upload(uint32_t id, uint32_t first, uint32_t count, const float *pos, const float *vel, const float *acc);
     int download(uint32_t id, int order, float *pos, float *col, float *vel, float *acc, uint32_t *ids);
+    // Stream-ordered variants for callers that stage through device memory
+    // themselves: state of particles [0, count) from d_stage (9 floats each:
+    // pos, vel, acc), and the scene's state in id order into d_out (12
+    // floats each: pos, colour, vel, acc) and d_ids.
+    hipStream_t stream() const { return stream_; }
+    int enqueue_upload(uint32_t id, const float *d_stage, uint32_t count);
+    int enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids);
 
     // Enqueues `substeps` substeps of every active scene and returns.
     int step(uint32_t substeps);
@@ -131,6 +138,14 @@ class Pool {
     std::vector<Consts> h_cs_;
     std::vector<uint32_t> h_chunk_;
     bool tables_dirty_ = true;
+    // pinned staging of the tables, a ring of kTableRing copies, each reused
+    // once the event of its last copy has completed
+    static constexpr int kTableRing = 4;
+    Consts *pin_cs_[kTableRing] = {};
+    uint32_t *pin_chunk_[kTableRing] = {};
+    hipEvent_t tables_ev_[kTableRing] = {};
+    bool tables_ev_pending_[kTableRing] = {};
+    int table_slot_ = 0;
 
     // captured substep
     hipGraph_t graph_ = nullptr;

@@ -136,6 +136,14 @@ void Pool::release() {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     prof_live_.clear();
     ev_pool_.clear();
+    for (int k = 0; k < kTableRing; ++k) {
+        if (pin_cs_[k]) (void)hipHostFree(pin_cs_[k]);
+        if (pin_chunk_[k]) (void)hipHostFree(pin_chunk_[k]);
+        if (tables_ev_[k]) (void)hipEventDestroy(tables_ev_[k]);
+        pin_cs_[k] = nullptr;
+        pin_chunk_[k] = nullptr;
+        tables_ev_[k] = nullptr;
+    }
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
 }
@@ -181,6 +189,13 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (!rc) rc = dalloc(&far_move_, scene_cap);
     if (!rc) rc = dalloc(&d_cs_, scene_cap);
     if (!rc) rc = dalloc(&d_chunk_, h_chunk_.size());
+    for (int k = 0; k < kTableRing && !rc; ++k) {
+        e = hipHostMalloc((void **)&pin_cs_[k], sizeof(Consts) * scene_cap, hipHostMallocDefault);
+        if (e == hipSuccess)
+            e = hipHostMalloc((void **)&pin_chunk_[k], sizeof(uint32_t) * h_chunk_.size(), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&tables_ev_[k], hipEventDisableTiming);
+        if (e != hipSuccess) rc = fail(LF_ERR_ALLOC, "pool: pinned staging: %s", hipGetErrorString(e));
+    }
     if (!rc) {
         // Counts start at zero; afterwards every scan clears them.
         e = hipMemsetAsync(counts_, 0, sizeof(uint32_t) * nbuckets_, stream_);
@@ -211,20 +226,31 @@ Consts Pool::make_scene_consts(uint32_t id) const {
     return c;
 }
 
-// Uploads the scene table and chunk map when they changed (stream-ordered).
+// Uploads the scene table (entries up to the highest live scene) and the
+// chunk map when they changed: stream-ordered copies from pinned staging,
+// which is reused once the previous copy from it has run.
 int Pool::push_tables() {
     if (!tables_dirty_) return LF_OK;
     std::fill(h_chunk_.begin(), h_chunk_.end(), kNoScene);
+    uint32_t top = 0;
     for (uint32_t id = 0; id < scenes_.size(); ++id) {
         const SceneRec &s = scenes_[id];
         if (!s.live) continue;
         h_cs_[id] = make_scene_consts(id);
+        top = id + 1;
         if (s.active)
             for (uint32_t k = 0; k < s.nchunks; ++k) h_chunk_[s.first / kChunk + k] = id;
     }
-    LF_HIP(hipMemcpyAsync(d_cs_, h_cs_.data(), sizeof(Consts) * h_cs_.size(), hipMemcpyHostToDevice, stream_));
-    LF_HIP(hipMemcpyAsync(d_chunk_, h_chunk_.data(), sizeof(uint32_t) * h_chunk_.size(), hipMemcpyHostToDevice,
+    const int k = table_slot_;
+    table_slot_ = (table_slot_ + 1) % kTableRing;
+    if (tables_ev_pending_[k]) LF_HIP(hipEventSynchronize(tables_ev_[k]));
+    std::memcpy(pin_cs_[k], h_cs_.data(), sizeof(Consts) * top);
+    std::memcpy(pin_chunk_[k], h_chunk_.data(), sizeof(uint32_t) * h_chunk_.size());
+    if (top) LF_HIP(hipMemcpyAsync(d_cs_, pin_cs_[k], sizeof(Consts) * top, hipMemcpyHostToDevice, stream_));
+    LF_HIP(hipMemcpyAsync(d_chunk_, pin_chunk_[k], sizeof(uint32_t) * h_chunk_.size(), hipMemcpyHostToDevice,
                           stream_));
+    LF_HIP(hipEventRecord(tables_ev_[k], stream_));
+    tables_ev_pending_[k] = true;
     tables_dirty_ = false;
     return LF_OK;
 }
@@ -475,7 +501,10 @@ void Pool::enqueue_substep() {
           [&](const Launch &L) { launch_xsph(L, pv, tune_.xsph, P1_, X_, V2_, start_, MS_, far_move_, V0_); });
 }
 
+// Launches of the executable graph may still be queued: wait for them before
+// destroying it.
 void Pool::drop_graph() {
+    if (graph_exec_ && stream_) (void)hipStreamSynchronize(stream_);
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     graph_exec_ = nullptr;
@@ -586,6 +615,20 @@ int Pool::upload(uint32_t id, uint32_t first, uint32_t count, const float *pos, 
     (void)hipFree(d_stage);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "upload: %s", hipGetErrorString(e));
     return LF_OK;
+}
+
+int Pool::enqueue_upload(uint32_t id, const float *d_stage, uint32_t count) {
+    const SceneRec *s = scene(id);
+    if (!s || count > s->n) return fail(LF_ERR_INVALID, "enqueue_upload: bad scene %u or count", id);
+    if (int rc = to_id_order(id)) return rc;
+    launch_upload(stream_, d_stage, s->first, 0, count, P0_, V0_, A0_, L0_);
+    return check_launch("enqueue_upload");
+}
+
+int Pool::enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids) {
+    if (!scene(id)) return fail(LF_ERR_INVALID, "enqueue_export: no scene %u", id);
+    launch_export(stream_, make_scene_consts(id), P0_, V0_, A0_, D_, true, d_out, d_ids);
+    return check_launch("enqueue_export");
 }
 
 int Pool::download(uint32_t id, int order, float *pos, float *col, float *vel, float *acc, uint32_t *ids) {

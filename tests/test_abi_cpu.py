@@ -36,19 +36,20 @@ def test_library_exports_every_declared_symbol():
     assert sorted(L.EXPORTED) == names  # the Python binding covers the whole ABI
 
 
-def test_abi_version_and_struct_layout(tmp_path):
-    """The ctypes mirror of lf_params matches the C header (size and offsets)."""
+@pytest.mark.parametrize("ctype,cname", [(L.LfParams, "lf_params"), (L.LfServerStats, "lf_server_stats")])
+def test_abi_version_and_struct_layout(tmp_path, ctype, cname):
+    """The ctypes mirrors of the ABI structs match the C header (size and offsets)."""
     assert L.lib.lf_version() == 1
-    fields = [f for f, _ in L.LfParams._fields_]
+    fields = [f for f, _ in ctype._fields_]
     src = tmp_path / "layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lf_abi.h"\nint main(void){'
-                   + 'printf("%zu", sizeof(lf_params));'
-                   + "".join(f'printf(" %zu", offsetof(lf_params, {f}));' for f in fields) + "return 0;}\n")
+                   + f'printf("%zu", sizeof({cname}));'
+                   + "".join(f'printf(" %zu", offsetof({cname}, {f}));' for f in fields) + "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert out[0] == C.sizeof(L.LfParams)
-    assert out[1:] == [getattr(L.LfParams, f).offset for f in fields]
+    assert out[0] == C.sizeof(ctype)
+    assert out[1:] == [getattr(ctype, f).offset for f in fields]
 
 
 def test_params_default_match_reference_init():
@@ -92,6 +93,24 @@ def test_batch_create_without_device_fails_cleanly():
     h = C.c_void_p()
     rc = L.lib.lf_batch_create(4096, 4096, 4, C.byref(h))
     assert rc < 0 and not h.value and L.lib.lf_last_error()
+
+
+def test_server_create_without_device_fails_cleanly():
+    if lf.device_count() > 0:
+        pytest.skip("a device is visible")
+    h = C.c_void_p()
+    assert L.lib.lf_server_create(4096, 4096, 4, L.SERVE_BACKFILL, C.byref(h)) < 0 and not h.value
+    assert L.lib.lf_server_create(4096, 4096, 4, 7, C.byref(h)) == L.LF_ERR_INVALID
+
+
+def test_served_workload_is_seeded():
+    from libfluid_amd.serve import served_jobs
+    a = served_jobs(20, rate=10.0, seed=3)
+    b = served_jobs(20, rate=10.0, seed=3)
+    assert [j.cfg.n for j in a] == [j.cfg.n for j in b] and [j.steps for j in a] == [j.steps for j in b]
+    t = np.array([j.arrival for j in a])
+    assert np.all(np.diff(t) > 0)
+    assert all(4096 <= j.cfg.n <= 262144 and 200 <= j.steps <= 2000 for j in a)
 
 
 def test_batch_footprint():
