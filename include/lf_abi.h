@@ -167,6 +167,34 @@ int lf_profile_read(lf_scene *s, uint64_t *launches, double *ms);
  * the event-timed duration in ms. */
 int lf_copy_probe(lf_scene *s, size_t bytes, int repeats, double *ms);
 
+/* ---- formats either side of the stepping path ------------------------ */
+/* Checkpoint: an exact binary snapshot (parameters, step count, last substep
+ * length, and per particle in id order the double-float position, velocity
+ * and acceleration) with a checksum.  A scene loaded from it continues
+ * bitwise as the saved scene would have.  The reference has no checkpoint;
+ * its round trip exportToData -> setupFromData (fluidsystem.cu:154-189) keeps
+ * FP32 positions and velocities only, and no acceleration. */
+int lf_checkpoint_save(lf_scene *s, const char *path);
+int lf_checkpoint_load(const char *path, lf_scene **out);
+/* Device-side setup: positions of particles [first, ...) generated on the
+ * GPU, the same values as lf_jittered_lattice / lf_fluid_block produce on the
+ * host (createFluidBlock, fluidsystem.cu:66-127, whose N single-point
+ * host-to-device copies, :121, this replaces).  Velocities and accelerations
+ * are left as they are. */
+int lf_setup_jittered_lattice(lf_scene *s, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float dx,
+                              const float origin[3], float jitter, uint64_t seed);
+int lf_setup_fluid_block(lf_scene *s, uint32_t first, uint32_t n, const float mn[3], const float mx[3],
+                         uint32_t *written, float *mass, float *particle_size);
+/* Asynchronous .geo export (the reference driver's dumpToGeo after
+ * exportToData, test/src/main.cpp:10-74,137-150, done synchronously there):
+ * the current state is snapshotted on the scene's stream, copied to host
+ * memory on a second stream and written by a background thread, in the given
+ * lf_order, while stepping goes on.  The files are byte-identical to
+ * lf_download + lf_write_geo at the same state.  lf_export_wait waits for
+ * every pending file and reports the first error since the last wait. */
+int lf_export_geo_async(lf_scene *s, const char *filebase, uint32_t cnt, int order);
+int lf_export_wait(lf_scene *s);
+
 /* ---- launch shapes (no counterpart in the reference, whose launch shape is
  *      derived from the largest cell occupancy every substep,
  *      fluidsystem.cu:319-323) ------------------------------------------- */
@@ -218,6 +246,15 @@ int lf_batch_sync(lf_batch *b);
 int lf_batch_scene_info(const lf_batch *b, uint32_t scene, uint32_t *num_particles, uint64_t *steps, int *active);
 int lf_batch_usage(const lf_batch *b, uint32_t *scenes, uint32_t *slots_used, uint32_t *slot_cap,
                    uint32_t *buckets_used, uint32_t *bucket_cap);
+/* The formats above, for one scene of a batch (checkpoint_load admits it). */
+int lf_batch_checkpoint_save(lf_batch *b, uint32_t scene, const char *path);
+int lf_batch_checkpoint_load(lf_batch *b, const char *path, uint32_t *scene);
+int lf_batch_setup_jittered_lattice(lf_batch *b, uint32_t scene, uint32_t first, uint32_t nx, uint32_t ny,
+                                    uint32_t nz, float dx, const float origin[3], float jitter, uint64_t seed);
+int lf_batch_setup_fluid_block(lf_batch *b, uint32_t scene, uint32_t first, uint32_t n, const float mn[3],
+                               const float mx[3], uint32_t *written, float *mass, float *particle_size);
+int lf_batch_export_geo_async(lf_batch *b, uint32_t scene, const char *filebase, uint32_t cnt, int order);
+int lf_batch_export_wait(lf_batch *b);
 int lf_batch_set_tuning(lf_batch *b, const int block_sizes[4]);
 int lf_batch_get_tuning(const lf_batch *b, int block_sizes[4]);
 int lf_batch_autotune(lf_batch *b, uint32_t reps, double *ms);

@@ -163,6 +163,21 @@ def _load():
         "lf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_profile_read": (C.c_int, [C.c_void_p, P(C.c_uint64), P(C.c_double)]),
+        "lf_checkpoint_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "lf_checkpoint_load": (C.c_int, [C.c_char_p, P(C.c_void_p)]),
+        "lf_setup_jittered_lattice": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                C.c_float, f32p, C.c_float, C.c_uint64]),
+        "lf_setup_fluid_block": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, f32p, f32p, u32p, f32p, f32p]),
+        "lf_export_geo_async": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_int]),
+        "lf_export_wait": (C.c_int, [C.c_void_p]),
+        "lf_batch_checkpoint_save": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p]),
+        "lf_batch_checkpoint_load": (C.c_int, [C.c_void_p, C.c_char_p, u32p]),
+        "lf_batch_setup_jittered_lattice": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                                      C.c_uint32, C.c_float, f32p, C.c_float, C.c_uint64]),
+        "lf_batch_setup_fluid_block": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, f32p, f32p, u32p,
+                                                 f32p, f32p]),
+        "lf_batch_export_geo_async": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_uint32, C.c_int]),
+        "lf_batch_export_wait": (C.c_int, [C.c_void_p]),
         "lf_server_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, P(C.c_void_p)]),
         "lf_server_destroy": (C.c_int, [C.c_void_p]),
         "lf_server_submit": (C.c_int, [C.c_void_p, P(LfParams), C.c_uint32, f32p, f32p, C.c_float, C.c_uint32,
@@ -195,7 +210,10 @@ EXPORTED = (
     "lf_batch_step lf_batch_sync lf_batch_scene_info lf_batch_usage lf_batch_set_tuning "
     "lf_batch_get_tuning lf_batch_autotune lf_batch_set_graphs lf_batch_profile_enable "
     "lf_batch_profile_read lf_server_create lf_server_destroy lf_server_submit lf_server_tick "
-    "lf_server_job_status lf_server_job_result lf_server_get_stats lf_server_take_finished lf_server_configure"
+    "lf_server_job_status lf_server_job_result lf_server_get_stats lf_server_take_finished lf_server_configure "
+    "lf_checkpoint_save lf_checkpoint_load lf_setup_jittered_lattice lf_setup_fluid_block lf_export_geo_async "
+    "lf_export_wait lf_batch_checkpoint_save lf_batch_checkpoint_load lf_batch_setup_jittered_lattice "
+    "lf_batch_setup_fluid_block lf_batch_export_geo_async lf_batch_export_wait"
 ).split()
 
 

@@ -95,16 +95,16 @@ void FluidSystem::createFluidBlock(const uint &firstPtIdx, const uint &numPoints
     }
     if (numPoints == 0) return;
     const float mn[3] = {min_x, min_y, min_z}, mx[3] = {max_x, max_y, max_z};
-    std::vector<float> pts(3 * (size_t)numPoints);
     uint32_t written = 0;
     float mass = 0.0f, psize = 0.0f;
-    if (!check(lf_fluid_block(numPoints, mn, mx, m_params.rest_density, pts.data(), &written, &mass, &psize),
+    // The lattice is generated on the device (the reference writes it point
+    // by point from the host, fluidsystem.cu:113-126).
+    if (!check(lf_setup_fluid_block(m_scene, firstPtIdx, numPoints, mn, mx, &written, &mass, &psize),
                "createFluidBlock"))
         return;
     // The block sets the (single) particle mass and size (fluidsystem.cu:105,108).
     setMass(mass);
     setParticleSize(psize);
-    if (written) check(lf_upload(m_scene, firstPtIdx, written, pts.data(), nullptr, nullptr), "createFluidBlock");
 }
 
 void FluidSystem::exportToData(std::vector<float3> &points, std::vector<float3> &colour,

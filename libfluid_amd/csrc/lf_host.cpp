@@ -3,6 +3,7 @@
 // Compiled with -ffp-contract=off so the FP32 lattice loops round exactly as
 // written.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -36,6 +37,23 @@ static inline uint64_t splitmix64(uint64_t &s) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+void block_axes(uint32_t n, const float mn[3], const float mx[3], float rest_density, std::vector<float> axes[3],
+                float *mass, float *particle_size) {
+    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
+    const float volume = dx * dy * dz;
+    const float part_volume = volume / float(n);
+    const float approx = std::cbrt(part_volume);
+    const float rx = dx / approx + 0.5f, ry = dy / approx + 0.5f, rz = dz / approx + 0.5f;
+    const float diam = std::min(std::min(dx / rx, dy / ry), dz / rz);
+    for (int a = 0; a < 3; ++a) {
+        axes[a].clear();
+        // The loops stop at n points in total, so no axis needs more than n values.
+        for (float v = mn[a] + 0.5f * diam; v <= mx[a] && axes[a].size() < n; v += diam) axes[a].push_back(v);
+    }
+    if (mass) *mass = part_volume * rest_density;
+    if (particle_size) *particle_size = diam * 0.5f;
 }
 
 // Uniform double in [0, 1) with 53 random bits.
@@ -115,16 +133,12 @@ int lf_fluid_block(uint32_t n, const float mn[3], const float mx[3], float rest_
                    uint32_t *written, float *mass, float *particle_size) {
     if (!mn || !mx || (!pos_xyz && n)) return lf::fail(LF_ERR_INVALID, "lf_fluid_block: null argument");
     if (n == 0) return lf::fail(LF_ERR_INVALID, "lf_fluid_block: zero points");
-    const float dx = mx[0] - mn[0], dy = mx[1] - mn[1], dz = mx[2] - mn[2];
-    const float volume = dx * dy * dz;
-    const float part_volume = volume / float(n);
-    const float approx = std::cbrt(part_volume);
-    const float rx = dx / approx + 0.5f, ry = dy / approx + 0.5f, rz = dz / approx + 0.5f;
-    const float diam = std::min(std::min(dx / rx, dy / ry), dz / rz);
+    std::vector<float> ax[3];
+    lf::block_axes(n, mn, mx, rest_density, ax, mass, particle_size);
     uint32_t cnt = 0;
-    for (float x = mn[0] + 0.5f * diam; x <= mx[0]; x += diam)
-        for (float y = mn[1] + 0.5f * diam; y <= mx[1]; y += diam)
-            for (float z = mn[2] + 0.5f * diam; z <= mx[2]; z += diam)
+    for (float x : ax[0])
+        for (float y : ax[1])
+            for (float z : ax[2])
                 if (cnt < n) {
                     pos_xyz[3 * cnt] = x;
                     pos_xyz[3 * cnt + 1] = y;
@@ -132,8 +146,6 @@ int lf_fluid_block(uint32_t n, const float mn[3], const float mx[3], float rest_
                     ++cnt;
                 }
     if (written) *written = cnt;
-    if (mass) *mass = part_volume * rest_density;
-    if (particle_size) *particle_size = diam * 0.5f;
     return LF_OK;
 }
 

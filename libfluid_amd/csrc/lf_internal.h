@@ -1,6 +1,8 @@
 // lf_internal.h -- shared host-side internals of libfluid_hip.so.
 #pragma once
 
+#include <vector>
+
 #include "lf_abi.h"
 
 namespace lf {
@@ -9,5 +11,12 @@ namespace lf {
 // returns `code`.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
+
+// createFluidBlock's lattice (fluidsystem.cu:66-127) as per-axis coordinate
+// lists: the FP32 running sums of its three loops (x outermost, z innermost);
+// the block is the first min(n, |ax| |ay| |az|) points of their product.
+// Also returns the mass (particle volume x rho0) and particle size.
+void block_axes(uint32_t n, const float mn[3], const float mx[3], float rest_density, std::vector<float> axes[3],
+                float *mass, float *particle_size);
 
 }  // namespace lf

@@ -62,6 +62,16 @@ void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, con
                  float4 *D2);
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
                    const float4 *D, bool by_id, float *out, uint32_t *ids);
+// Exact state (12 floats per particle: P.xyz, L.xyz, V.xyz, A.xyz), id order.
+void launch_state_out(hipStream_t st, const Consts &c, const float4 *P, const float4 *Lw, const float4 *V,
+                      const float4 *A, float *out);
+void launch_state_in(hipStream_t st, uint32_t first, uint32_t count, const float *in, float4 *P, float4 *Lw,
+                     float4 *V, float4 *A);
+// Device-side setup of positions (ids id0.., slot first + id).
+void launch_setup_axes(hipStream_t st, uint32_t first, uint32_t id0, uint32_t count, uint32_t ny, uint32_t nz,
+                       const float *ax, const float *ay, const float *az, float4 *P, float4 *Lw);
+void launch_setup_jitter(hipStream_t st, uint32_t first, uint32_t id0, uint32_t nx, uint32_t ny, uint32_t nz,
+                         float dx, const float origin[3], float jitter, uint64_t seed, float4 *P, float4 *Lw);
 void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t *cnt);
 void launch_nb_fill(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, const uint64_t *off,
                     uint32_t *list);

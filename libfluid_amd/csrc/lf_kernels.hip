@@ -722,6 +722,85 @@ __global__ __launch_bounds__(kBlock) void k_export(Consts c, const float4 *__res
     ids[dst] = id;
 }
 
+// Exact state in particle-id order, 12 floats per particle: position high
+// words, position low words, velocity, acceleration (checkpoints).
+__global__ __launch_bounds__(kBlock) void k_state_out(Consts c, const float4 *__restrict__ P,
+                                                      const float4 *__restrict__ L, const float4 *__restrict__ V,
+                                                      const float4 *__restrict__ A, float *__restrict__ out) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= c.count) return;
+    const uint32_t s = c.first + k;
+    const float4 p = P[s], l = L[s], v = V[s], a = A[s];
+    float *o = out + 12 * (size_t)__float_as_uint(p.w);
+    o[0] = p.x; o[1] = p.y; o[2] = p.z;
+    o[3] = l.x; o[4] = l.y; o[5] = l.z;
+    o[6] = v.x; o[7] = v.y; o[8] = v.z;
+    o[9] = a.x; o[10] = a.y; o[11] = a.z;
+}
+
+// Inverse of k_state_out into a scene in id order (slot first + id).
+__global__ __launch_bounds__(kBlock) void k_state_in(uint32_t first, uint32_t count, const float *__restrict__ in,
+                                                     float4 *__restrict__ P, float4 *__restrict__ L,
+                                                     float4 *__restrict__ V, float4 *__restrict__ A) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const float *i = in + 12 * (size_t)k;
+    const uint32_t s = first + k;
+    P[s] = make_float4(i[0], i[1], i[2], __uint_as_float(k));
+    L[s] = make_float4(i[3], i[4], i[5], 0.0f);
+    V[s] = make_float4(i[6], i[7], i[8], 0.0f);
+    A[s] = make_float4(i[9], i[10], i[11], 0.0f);
+}
+
+// Device-side scene setup.  Both write positions (and clear the low words)
+// of particle ids [id0, id0 + count) of a scene in id order; velocities and
+// accelerations are left as they are, as the reference's createFluidBlock
+// does (fluidsystem.cu:113-126).
+//
+// Lattice from per-axis coordinate lists (x outermost, z innermost): the
+// lists are the FP32 running sums of createFluidBlock's loops.
+__global__ __launch_bounds__(kBlock) void k_setup_axes(uint32_t first, uint32_t id0, uint32_t count, uint32_t ny,
+                                                       uint32_t nz, const float *__restrict__ ax,
+                                                       const float *__restrict__ ay, const float *__restrict__ az,
+                                                       float4 *__restrict__ P, float4 *__restrict__ L) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t l = k % nz, j = (k / nz) % ny, i = k / (nz * ny);
+    const uint32_t id = id0 + k;
+    P[first + id] = make_float4(ax[i], ay[j], az[l], __uint_as_float(id));
+    L[first + id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Jittered lattice, the same values as the host generator: the m-th
+// SplitMix64 output (m = 3 point + axis + 1, counter-based) gives the
+// jitter of one coordinate, evaluated in FP64 without contraction.
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t m) {
+    uint64_t z = seed + m * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_setup_jitter(uint32_t first, uint32_t id0, uint32_t nx, uint32_t ny,
+                                                         uint32_t nz, float dx, float ox, float oy, float oz,
+                                                         float jitter, uint64_t seed, float4 *__restrict__ P,
+                                                         float4 *__restrict__ L) {
+#pragma clang fp contract(off)
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= nx * ny * nz) return;
+    const uint32_t idx[3] = {k / (ny * nz), (k / nz) % ny, k % nz};
+    const float org[3] = {ox, oy, oz};
+    float v[3];
+    for (int a = 0; a < 3; ++a) {
+        const uint64_t z = splitmix64_at(seed, 3ull * k + (uint64_t)a + 1ull);
+        const double u = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+        v[a] = (float)((double)org[a] + ((double)idx[a] + 0.5) * (double)dx + u * (double)jitter);
+    }
+    const uint32_t id = id0 + k;
+    P[first + id] = make_float4(v[0], v[1], v[2], __uint_as_float(id));
+    L[first + id] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
 // Neighbour sets of the density pass (debug): count, then fill; indexed by
 // the scene-local slot.
 __global__ __launch_bounds__(kBlock) void k_nb_count(Consts c, const float4 *__restrict__ Ps,
@@ -886,10 +965,37 @@ void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, con
 }
 
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *D, bool by_id, float *out, uint32_t *ids) {
+                   const float4 *D, bool by_id, float *out, uint32_t *ids) {  // ids may not be null
     if (c.count)
         hipLaunchKernelGGL(k_export, dim3(blocks_for(c.count)), dim3(kBlock), 0, st, c, P, V, A, D, by_id ? 1 : 0,
                            out, ids);
+}
+
+void launch_state_out(hipStream_t st, const Consts &c, const float4 *P, const float4 *Lw, const float4 *V,
+                      const float4 *A, float *out) {
+    if (c.count)
+        hipLaunchKernelGGL(k_state_out, dim3(blocks_for(c.count)), dim3(kBlock), 0, st, c, P, Lw, V, A, out);
+}
+
+void launch_state_in(hipStream_t st, uint32_t first, uint32_t count, const float *in, float4 *P, float4 *Lw,
+                     float4 *V, float4 *A) {
+    if (count)
+        hipLaunchKernelGGL(k_state_in, dim3(blocks_for(count)), dim3(kBlock), 0, st, first, count, in, P, Lw, V, A);
+}
+
+void launch_setup_axes(hipStream_t st, uint32_t first, uint32_t id0, uint32_t count, uint32_t ny, uint32_t nz,
+                       const float *ax, const float *ay, const float *az, float4 *P, float4 *Lw) {
+    if (count)
+        hipLaunchKernelGGL(k_setup_axes, dim3(blocks_for(count)), dim3(kBlock), 0, st, first, id0, count, ny, nz, ax,
+                           ay, az, P, Lw);
+}
+
+void launch_setup_jitter(hipStream_t st, uint32_t first, uint32_t id0, uint32_t nx, uint32_t ny, uint32_t nz,
+                         float dx, const float origin[3], float jitter, uint64_t seed, float4 *P, float4 *Lw) {
+    const size_t n = (size_t)nx * ny * nz;
+    if (n)
+        hipLaunchKernelGGL(k_setup_jitter, dim3(blocks_for(n)), dim3(kBlock), 0, st, first, id0, nx, ny, nz, dx,
+                           origin[0], origin[1], origin[2], jitter, seed, P, Lw);
 }
 
 void launch_nb_count(hipStream_t st, const Consts &c, const float4 *Ps, const uint32_t *start, uint32_t *cnt) {

@@ -14,6 +14,7 @@
 
 #include "lf_abi.h"
 #include "lf_device.h"
+#include "lf_internal.h"
 #include "lf_kernels.h"
 
 namespace lf {
@@ -68,7 +69,25 @@ class Pool {
     // floats each: pos, colour, vel, acc) and d_ids.
     hipStream_t stream() const { return stream_; }
     int enqueue_upload(uint32_t id, const float *d_stage, uint32_t count);
-    int enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids);
+    int enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids, bool by_id = true);
+
+    // Exact state of one scene in id order, 12 floats per particle (position
+    // high and low words, velocity, acceleration); synchronous.  load_state
+    // also sets the scene's step count.
+    int save_state(uint32_t id, std::vector<float> &out);
+    int load_state(uint32_t id, const float *in, uint64_t steps);
+    // Device-side setup of particle positions (ids [id0, ...)).
+    int setup_jitter(uint32_t id, uint32_t id0, uint32_t nx, uint32_t ny, uint32_t nz, float dx, const float o[3],
+                     float jitter, uint64_t seed);
+    int setup_block(uint32_t id, uint32_t id0, uint32_t n, const float mn[3], const float mx[3], uint32_t *written,
+                    float *mass, float *psize);
+    // Asynchronous .geo export of one scene's current state (cell or id
+    // order): a snapshot is taken on the stream, copied to host memory on a
+    // second stream and written by a background thread while stepping goes
+    // on.  export_wait waits for every pending file and reports the first
+    // error since the last wait.
+    int export_geo_async(uint32_t id, const char *filebase, uint32_t cnt, int order);
+    int export_wait();
 
     // Enqueues `substeps` substeps of every active scene and returns.
     int step(uint32_t substeps);
@@ -91,6 +110,8 @@ class Pool {
     void set_graphs(bool on);
 
   private:
+    struct Exporter;
+    void stop_exporter();
     struct ProfRec {
         int kind;
         hipEvent_t a, b;
@@ -152,6 +173,8 @@ class Pool {
     hipGraphExec_t graph_exec_ = nullptr;
     bool graph_normals_ = false;  // the captured substep has the normals pass
 
+    Exporter *exporter_ = nullptr;
+
     // profiling
     uint32_t prof_mask_ = 0;
     std::vector<ProfRec> prof_live_;
@@ -161,5 +184,17 @@ class Pool {
 // Kernel constants of a scene with parameters p, substep dt and the given
 // debug flags (placement fields left zero).
 Consts make_consts(const lf_params &p, float dt, int debug_flags);
+
+template <class T>
+int Pool::dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LF_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    return LF_OK;
+}
 
 }  // namespace lf

@@ -110,19 +110,8 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
 // ---------------------------------------------------------------------------
 // lifetime
 
-template <class T>
-int Pool::dalloc(T **p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(LF_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-    }
-    return LF_OK;
-}
-
 void Pool::release() {
+    stop_exporter();
     drop_graph();
     void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, M_.w32, M_.w64, MS_.w32,
                     MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, far_move_, d_cs_,
@@ -625,9 +614,9 @@ int Pool::enqueue_upload(uint32_t id, const float *d_stage, uint32_t count) {
     return check_launch("enqueue_upload");
 }
 
-int Pool::enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids) {
+int Pool::enqueue_export(uint32_t id, float *d_out, uint32_t *d_ids, bool by_id) {
     if (!scene(id)) return fail(LF_ERR_INVALID, "enqueue_export: no scene %u", id);
-    launch_export(stream_, make_scene_consts(id), P0_, V0_, A0_, D_, true, d_out, d_ids);
+    launch_export(stream_, make_scene_consts(id), P0_, V0_, A0_, D_, by_id, d_out, d_ids);
     return check_launch("enqueue_export");
 }
 

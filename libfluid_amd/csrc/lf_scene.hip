@@ -11,6 +11,7 @@
 #include <new>
 
 #include "lf_internal.h"
+#include "lf_io.h"
 #include "lf_pool.h"
 
 using lf::fail;
@@ -188,6 +189,59 @@ int lf_set_graphs(lf_scene *s, int on) {
     return LF_OK;
 }
 
+// ---- formats either side of the path ----------------------------------------
+
+int lf_checkpoint_save(lf_scene *s, const char *path) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    std::vector<float> v;
+    if (int rc = s->pool.save_state(s->id, v)) return rc;
+    const lf::SceneRec *r = s->pool.scene(s->id);
+    return lf::write_checkpoint(path, r->params, r->n, r->steps, r->dt, v);
+}
+
+int lf_checkpoint_load(const char *path, lf_scene **out) {
+    if (!out) return fail(LF_ERR_INVALID, "lf_checkpoint_load: null output handle");
+    *out = nullptr;
+    lf_params p;
+    uint32_t n = 0;
+    uint64_t steps = 0;
+    float dt = 0.0f;
+    std::vector<float> v;
+    if (int rc = lf::read_checkpoint(path, &p, &n, &steps, &dt, v)) return rc;
+    lf_scene *s = nullptr;
+    if (int rc = lf_create(&p, n, &s)) return rc;
+    int rc = s->pool.load_state(s->id, v.data(), steps);
+    if (!rc && dt > 0.0f) rc = s->pool.set_dt(s->id, dt);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return LF_OK;
+}
+
+int lf_setup_jittered_lattice(lf_scene *s, uint32_t first, uint32_t nx, uint32_t ny, uint32_t nz, float dx,
+                              const float origin[3], float jitter, uint64_t seed) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    return s->pool.setup_jitter(s->id, first, nx, ny, nz, dx, origin, jitter, seed);
+}
+
+int lf_setup_fluid_block(lf_scene *s, uint32_t first, uint32_t n, const float mn[3], const float mx[3],
+                         uint32_t *written, float *mass, float *particle_size) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    return s->pool.setup_block(s->id, first, n, mn, mx, written, mass, particle_size);
+}
+
+int lf_export_geo_async(lf_scene *s, const char *filebase, uint32_t cnt, int order) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    return s->pool.export_geo_async(s->id, filebase, cnt, order);
+}
+
+int lf_export_wait(lf_scene *s) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    return s->pool.export_wait();
+}
+
 // ---- lf_batch ---------------------------------------------------------------
 
 int lf_batch_footprint(uint32_t max_particles, uint32_t max_cells, uint32_t max_scenes, uint64_t *bytes) {
@@ -308,6 +362,54 @@ int lf_batch_set_graphs(lf_batch *b, int on) {
     if (!b) return fail(LF_ERR_INVALID, "null batch");
     b->pool.set_graphs(on != 0);
     return LF_OK;
+}
+
+int lf_batch_checkpoint_save(lf_batch *b, uint32_t scene, const char *path) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    std::vector<float> v;
+    if (int rc = b->pool.save_state(scene, v)) return rc;
+    const lf::SceneRec *r = b->pool.scene(scene);
+    return lf::write_checkpoint(path, r->params, r->n, r->steps, r->dt, v);
+}
+
+int lf_batch_checkpoint_load(lf_batch *b, const char *path, uint32_t *scene) {
+    if (!b || !scene) return fail(LF_ERR_INVALID, "null argument");
+    lf_params p;
+    uint32_t n = 0;
+    uint64_t steps = 0;
+    float dt = 0.0f;
+    std::vector<float> v;
+    if (int rc = lf::read_checkpoint(path, &p, &n, &steps, &dt, v)) return rc;
+    uint32_t id = 0;
+    if (int rc = b->pool.add_scene(p, n, dt, &id)) return rc;
+    if (int rc = b->pool.load_state(id, v.data(), steps)) {
+        (void)b->pool.remove_scene(id);
+        return rc;
+    }
+    *scene = id;
+    return LF_OK;
+}
+
+int lf_batch_setup_jittered_lattice(lf_batch *b, uint32_t scene, uint32_t first, uint32_t nx, uint32_t ny,
+                                    uint32_t nz, float dx, const float origin[3], float jitter, uint64_t seed) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    return b->pool.setup_jitter(scene, first, nx, ny, nz, dx, origin, jitter, seed);
+}
+
+int lf_batch_setup_fluid_block(lf_batch *b, uint32_t scene, uint32_t first, uint32_t n, const float mn[3],
+                               const float mx[3], uint32_t *written, float *mass, float *particle_size) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    return b->pool.setup_block(scene, first, n, mn, mx, written, mass, particle_size);
+}
+
+int lf_batch_export_geo_async(lf_batch *b, uint32_t scene, const char *filebase, uint32_t cnt, int order) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    return b->pool.export_geo_async(scene, filebase, cnt, order);
+}
+
+int lf_batch_export_wait(lf_batch *b) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    return b->pool.export_wait();
 }
 
 int lf_batch_profile_enable(lf_batch *b, int mask) {

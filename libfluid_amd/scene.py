@@ -158,6 +158,43 @@ class Scene:
         check(lib.lf_copy_probe(self._h, int(nbytes), int(repeats), C.byref(ms)), "lf_copy_probe")
         return ms.value
 
+    # -- formats either side of the path -------------------------------------
+    def checkpoint_save(self, path: str) -> None:
+        check(lib.lf_checkpoint_save(self._h, str(path).encode()), "lf_checkpoint_save")
+
+    @classmethod
+    def checkpoint_load(cls, path: str) -> "Scene":
+        self = cls.__new__(cls)
+        self._h = C.c_void_p()
+        check(lib.lf_checkpoint_load(str(path).encode(), C.byref(self._h)), "lf_checkpoint_load")
+        n = C.c_uint32(0)
+        check(lib.lf_num_particles(self._h, C.byref(n)), "lf_num_particles")
+        self.n = n.value
+        self.params = LfParams()
+        check(lib.lf_get_params(self._h, C.byref(self.params)), "lf_get_params")
+        return self
+
+    def setup_jittered_lattice(self, nx: int, ny: int, nz: int, dx: float, origin, jitter: float, seed: int,
+                               first: int = 0) -> None:
+        o = np.ascontiguousarray(origin, dtype=np.float32)
+        check(lib.lf_setup_jittered_lattice(self._h, int(first), int(nx), int(ny), int(nz), C.c_float(dx), fptr(o),
+                                            C.c_float(jitter), C.c_uint64(seed)), "lf_setup_jittered_lattice")
+
+    def setup_fluid_block(self, n: int, mn, mx, first: int = 0) -> tuple:
+        """(points written, mass, particle size)"""
+        a = np.ascontiguousarray(mn, dtype=np.float32)
+        b = np.ascontiguousarray(mx, dtype=np.float32)
+        w, m, s = C.c_uint32(0), C.c_float(0.0), C.c_float(0.0)
+        check(lib.lf_setup_fluid_block(self._h, int(first), int(n), fptr(a), fptr(b), C.byref(w), C.byref(m),
+                                       C.byref(s)), "lf_setup_fluid_block")
+        return w.value, m.value, s.value
+
+    def export_geo_async(self, filebase: str, cnt: int, order: int = L.LF_ORDER_CELL) -> None:
+        check(lib.lf_export_geo_async(self._h, str(filebase).encode(), int(cnt), int(order)), "lf_export_geo_async")
+
+    def export_wait(self) -> None:
+        check(lib.lf_export_wait(self._h), "lf_export_wait")
+
     # -- launch shapes ------------------------------------------------------
     def set_tuning(self, block_sizes) -> None:
         """Workgroup sizes of the density, normals, forces and XSPH passes."""
@@ -276,6 +313,31 @@ class Batch:
         v = [C.c_uint32(0) for _ in range(5)]
         check(lib.lf_batch_usage(self._h, *[C.byref(x) for x in v]), "lf_batch_usage")
         return dict(zip(("scenes", "slots_used", "slot_cap", "buckets_used", "bucket_cap"), [x.value for x in v]))
+
+    def checkpoint_save(self, scene: int, path: str) -> None:
+        check(lib.lf_batch_checkpoint_save(self._h, int(scene), str(path).encode()), "lf_batch_checkpoint_save")
+
+    def checkpoint_load(self, path: str) -> int:
+        sid = C.c_uint32(0)
+        check(lib.lf_batch_checkpoint_load(self._h, str(path).encode(), C.byref(sid)), "lf_batch_checkpoint_load")
+        n = C.c_uint32(0)
+        check(lib.lf_batch_scene_info(self._h, sid.value, C.byref(n), None, None), "lf_batch_scene_info")
+        self._n[sid.value] = n.value
+        return sid.value
+
+    def setup_jittered_lattice(self, scene: int, nx: int, ny: int, nz: int, dx: float, origin, jitter: float,
+                               seed: int, first: int = 0) -> None:
+        o = np.ascontiguousarray(origin, dtype=np.float32)
+        check(lib.lf_batch_setup_jittered_lattice(self._h, int(scene), int(first), int(nx), int(ny), int(nz),
+                                                  C.c_float(dx), fptr(o), C.c_float(jitter), C.c_uint64(seed)),
+              "lf_batch_setup_jittered_lattice")
+
+    def export_geo_async(self, scene: int, filebase: str, cnt: int, order: int = L.LF_ORDER_CELL) -> None:
+        check(lib.lf_batch_export_geo_async(self._h, int(scene), str(filebase).encode(), int(cnt), int(order)),
+              "lf_batch_export_geo_async")
+
+    def export_wait(self) -> None:
+        check(lib.lf_batch_export_wait(self._h), "lf_batch_export_wait")
 
     def set_tuning(self, block_sizes) -> None:
         bs = (C.c_int * 4)(*[int(b) for b in block_sizes])

@@ -113,6 +113,47 @@ def test_served_workload_is_seeded():
     assert all(4096 <= j.cfg.n <= 262144 and 200 <= j.steps <= 2000 for j in a)
 
 
+def _checkpoint_bytes(params, n, steps=7, dt=1e-4, corrupt=False):
+    """A checkpoint file: 136-byte header (magic, ABI version, count, steps,
+    dt, reserved, lf_params, 4 bytes of padding), 12 floats per particle and
+    the FNV-1a 64 of everything before."""
+    import struct
+    head = struct.pack("<8sIIQfI", b"LFCKPT01", 1, n, steps, dt, 0) + bytes(params) + b"\0" * 4
+    assert len(head) == 136
+    body = np.arange(12 * n, dtype=np.float32).tobytes()
+    h = 0xCBF29CE484222325
+    for byte in head + body:
+        h = ((h ^ byte) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return head + body + struct.pack("<Q", h ^ (1 if corrupt else 0))
+
+
+@pytest.mark.parametrize("kind,needle", [("magic", b"not a checkpoint"), ("checksum", b"checksum"),
+                                         ("truncated", b"truncated"), ("missing", b"cannot open")])
+def test_checkpoint_file_validation(tmp_path, kind, needle):
+    p = default_params(8)
+    data = _checkpoint_bytes(p, 10, corrupt=(kind == "checksum"))
+    if kind == "magic":
+        data = b"XXXXXXXX" + data[8:]
+    elif kind == "truncated":
+        data = data[:-20]
+    path = tmp_path / "c.lfck"
+    if kind != "missing":
+        path.write_bytes(data)
+    h = C.c_void_p()
+    assert L.lib.lf_checkpoint_load(str(path).encode(), C.byref(h)) == L.LF_ERR_INVALID and not h.value
+    assert needle in L.lib.lf_last_error()
+
+
+def test_checkpoint_valid_file_reaches_scene_creation(tmp_path):
+    if lf.device_count() > 0:
+        pytest.skip("a device is visible")
+    path = tmp_path / "c.lfck"
+    path.write_bytes(_checkpoint_bytes(default_params(8), 10))
+    h = C.c_void_p()
+    rc = L.lib.lf_checkpoint_load(str(path).encode(), C.byref(h))
+    assert rc < 0 and rc != L.LF_ERR_INVALID and not h.value  # the file passed; no device to load it on
+
+
 def test_batch_footprint():
     """Bytes a batch allocates: 444 per slot (slots rounded up to 512), 8 per
     bucket, a scene-table entry per scene, 4 per 512-slot chunk."""
