@@ -202,10 +202,19 @@ int lf_export_wait(lf_scene *s);
  * 256 or 512 (default 256).  Results do not depend on them. */
 int lf_set_tuning(lf_scene *s, const int block_sizes[4]);
 int lf_get_tuning(const lf_scene *s, int block_sizes[4]);
+/* Walk modes of the normals, forces and XSPH passes: 1 walks the density
+ * pass's neighbour masks as one flattened stream of set bits (in waves whose
+ * stencil rows all fit 64-slot masks), 0 walks them row by row.  Default
+ * {1, 1, 0} (measured fastest for the force walk's arithmetic-heavy steps and
+ * the XSPH walk's gather-heavy ones); lf_autotune picks per pass.  Results do
+ * not depend on them. */
+int lf_set_walks(lf_scene *s, const int flat[3]);
+int lf_get_walks(const lf_scene *s, int flat[3]);
 /* Times each block size on the scene's current state (`reps` substeps each,
- * state restored) and keeps the fastest per pass.  ms (may be NULL) receives
- * 12 averages: pass-major, block sizes 128, 256, 512.  Needs one lf_step first
- * (the substep length). */
+ * state restored) and keeps the fastest per pass; then, at those block
+ * sizes, both walk modes, keeping the faster per pass.  ms (may be NULL)
+ * receives the 12 block-size averages: pass-major, block sizes 128, 256, 512.
+ * Needs one lf_step first (the substep length). */
 int lf_autotune(lf_scene *s, uint32_t reps, double *ms);
 /* On (default): lf_step replays one captured hipGraph per substep; off: every
  * kernel is launched individually.  Profiling (lf_profile_enable) always
@@ -257,6 +266,8 @@ int lf_batch_export_geo_async(lf_batch *b, uint32_t scene, const char *filebase,
 int lf_batch_export_wait(lf_batch *b);
 int lf_batch_set_tuning(lf_batch *b, const int block_sizes[4]);
 int lf_batch_get_tuning(const lf_batch *b, int block_sizes[4]);
+int lf_batch_set_walks(lf_batch *b, const int flat[3]);
+int lf_batch_get_walks(const lf_batch *b, int flat[3]);
 int lf_batch_autotune(lf_batch *b, uint32_t reps, double *ms);
 int lf_batch_set_graphs(lf_batch *b, int on);
 int lf_batch_profile_enable(lf_batch *b, int mask);

@@ -141,6 +141,8 @@ def _load():
         "lf_write_geo": (C.c_int, [C.c_char_p, C.c_uint32, f32p, f32p, f32p, C.c_uint32, C.c_float]),
         "lf_set_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lf_get_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_set_walks": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_get_walks": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lf_autotune": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "lf_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_footprint": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_uint64)]),
@@ -159,6 +161,8 @@ def _load():
         "lf_batch_usage": (C.c_int, [C.c_void_p, u32p, u32p, u32p, u32p, u32p]),
         "lf_batch_set_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lf_batch_get_tuning": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_batch_set_walks": (C.c_int, [C.c_void_p, P(C.c_int)]),
+        "lf_batch_get_walks": (C.c_int, [C.c_void_p, P(C.c_int)]),
         "lf_batch_autotune": (C.c_int, [C.c_void_p, C.c_uint32, P(C.c_double)]),
         "lf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
         "lf_batch_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -204,11 +208,11 @@ EXPORTED = (
     "lf_destroy lf_num_particles lf_set_params lf_get_params lf_upload lf_download lf_step "
     "lf_sync lf_step_count lf_get_field lf_set_debug lf_neighbours lf_profile_enable "
     "lf_profile_read lf_copy_probe lf_fluid_block lf_jittered_lattice lf_uniform_points "
-    "lf_write_geo lf_set_tuning lf_get_tuning lf_autotune lf_set_graphs "
+    "lf_write_geo lf_set_tuning lf_get_tuning lf_set_walks lf_get_walks lf_autotune lf_set_graphs "
     "lf_batch_footprint lf_batch_create lf_batch_destroy lf_batch_add lf_batch_remove "
     "lf_batch_set_params lf_batch_set_dt lf_batch_set_active lf_batch_upload lf_batch_download "
     "lf_batch_step lf_batch_sync lf_batch_scene_info lf_batch_usage lf_batch_set_tuning "
-    "lf_batch_get_tuning lf_batch_autotune lf_batch_set_graphs lf_batch_profile_enable "
+    "lf_batch_get_tuning lf_batch_set_walks lf_batch_get_walks lf_batch_autotune lf_batch_set_graphs lf_batch_profile_enable "
     "lf_batch_profile_read lf_server_create lf_server_destroy lf_server_submit lf_server_tick "
     "lf_server_job_status lf_server_job_result lf_server_get_stats lf_server_take_finished lf_server_configure "
     "lf_checkpoint_save lf_checkpoint_load lf_setup_jittered_lattice lf_setup_fluid_block lf_export_geo_async "

@@ -160,6 +160,57 @@ __device__ __forceinline__ T at32(const T *__restrict__ base, uint32_t j) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + j * (uint32_t)sizeof(T));
 }
 
+// Raw buffer view of a particle array of n float4 slots (range-checked: a
+// load at or past the end returns zeros).  Loads through it take a per-lane
+// byte offset and a wave-uniform one (soffset), so a candidate loop whose
+// slot offset runs in step across the wave advances its address with scalar
+// adds.  0x00020000 is the gfx9 raw-buffer descriptor word 3 (32-bit data
+// format, no swizzle).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_buffer(const float4 *base, uint32_t n) {
+    const uint64_t bytes = (uint64_t)n * sizeof(float4);
+    const int records = bytes >= 0xFFFFFFFFull ? (int)0xFFFFFFFFu : (int)(uint32_t)bytes;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(base), (short)0, records, 0x00020000);
+}
+// x, y, z of the slot at byte offset voff + soff (soff wave-uniform).
+__device__ __forceinline__ float4 load_xyz(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)voff, (int)soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
+}
+
+// Largest v (v < 2^BITS) over the wave's active lanes, by BITS ballots.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+    uint32_t mx = 0u;
+#pragma unroll
+    for (int b = BITS - 1; b >= 0; --b) {
+        const uint32_t t = mx | (1u << b);
+        if (__ballot(v >= t)) mx = t;
+    }
+    return mx;
+}
+
+// Lane-mask forms of per-lane predicates.  lanes(p) is the wave's mask of
+// lanes where p holds (a compare lowers to a single v_cmp into it); combining
+// masks is scalar work.  shift_in and keep take the mask directly: a per-lane
+// bool built from an and of two compares would otherwise be materialised as
+// 0/1 in a vector register and compared again.
+__device__ __forceinline__ uint64_t lanes(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// m = 2 m + (this lane's bit of mask), one v_addc_co_u32 with the mask as
+// carry-in.
+__device__ __forceinline__ uint32_t shift_in(uint32_t m, uint64_t mask) {
+    uint64_t carry_out;
+    asm("v_addc_co_u32 %0, %1, %0, %0, %2" : "+v"(m), "=s"(carry_out) : "s"(mask));
+    return m;
+}
+
+// this lane's bit of mask ? v : 0, one v_cndmask_b32.
+__device__ __forceinline__ float keep(float v, uint64_t mask) {
+    float r;
+    asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(mask));
+    return r;
+}
+
 // Fast reciprocal / reciprocal square root (v_rcp_f32 / v_rsq_f32, ~1 ulp).
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }

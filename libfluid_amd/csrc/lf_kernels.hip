@@ -249,6 +249,10 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // scanned in full.  With use == false every row is scanned.  Each step takes
 // two set bits and issues both candidates' loads (ld) before using them
 // (fn, in slot order), so two gathers per lane are in flight at a time.
+// fn(cand, slot, ok) adds nothing when ok is false (a step with one set bit
+// left passes its first candidate again with ok = false); calling it
+// unconditionally keeps the second candidate's loads ahead of the first
+// candidate's arithmetic.
 template <class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
                                            uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
@@ -273,14 +277,93 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
                 m &= m - 1ull;  // stays 0 when it was 0
                 const auto a1 = ld(j1);
                 const auto a2 = ld(j2);
-                fn(a1, j1);
-                if (two) fn(a2, j2);
+                fn(a1, j1, true);
+                fn(a2, j2, two);
             }
         } else {
             for (uint32_t j = b[r]; j < e[r]; ++j)
-                if (j != i) fn(ld(j), j);
+                if (j != i) fn(ld(j), j, true);
         }
     }
+}
+
+// Flattened walk of the nine rows' masks.  Iterating row by row costs every
+// row the longest run of any lane in the wave; one stream of set bits over
+// all rows costs only the largest total.  Each lane splits its rows into
+// pseudo-rows of at most 32 slots (a row of 33..64 slots is two: its 64-bit
+// mask's low and high words, the second starting 32 slots later), compacts
+// the non-empty ones, (mask, first slot), into its own column of the
+// workgroup's LDS table tab[kFlatEntries][BS] and walks them in row order,
+// two set bits per step (the two may come from adjacent pseudo-rows).  The
+// next entry is fetched one entry ahead, so a row change does not wait on
+// LDS.  Visits the same slots in the same order as walk_masks.  Returns
+// false, having visited nothing, when a lane of the wave has a row of more
+// than 64 slots or more than kFlatEntries non-empty pseudo-rows.
+constexpr int kFlatEntries = 16;
+template <int BS, class Ld, class Fn>
+__device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
+                                          const RowMasks &mask, uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
+    const uint32_t t = threadIdx.x;
+    // An empty pseudo-row is written at the running count and overwritten by
+    // the next; entries at or past cnt are never used.
+    uint32_t cnt = 0u;
+    bool fits = true;
+    auto put = [&](uint32_t m, uint32_t base) {
+        tab[(cnt < (uint32_t)kFlatEntries ? cnt : 0u) * BS + t] = make_uint2(m, base);
+        cnt += m != 0u ? 1u : 0u;
+    };
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t len = e[r] - b[r];
+        const size_t at = (size_t)r * nslots + i;
+        if (len <= 32u) {
+            put(mask.w32[at], b[r]);
+        } else if (len <= 64u) {
+            const uint64_t m = mask.w64[at];
+            put((uint32_t)m, b[r]);
+            put((uint32_t)(m >> 32), b[r] + 32u);
+        } else {
+            fits = false;
+        }
+    }
+    if (__ballot(!fits || cnt > (uint32_t)kFlatEntries)) return false;
+    const uint2 e0 = tab[t];
+    uint32_t m = cnt > 0u ? e0.x : 0u, base = e0.y;
+    uint2 nx = tab[BS + t];  // entry nxt as read: used only if nxt < cnt
+    uint32_t nxt = 1u;
+    auto advance = [&]() {  // the current pseudo-row is used up: take entry nxt, fetch the one after
+        m = nxt < cnt ? nx.x : 0u;
+        base = nx.y;
+        ++nxt;
+        nx = tab[(nxt < (uint32_t)kFlatEntries ? nxt : 0u) * BS + t];
+    };
+    while (m) {
+        const uint32_t j1 = base + (uint32_t)__builtin_ctz(m);
+        m &= m - 1u;
+        if (!m) advance();
+        const bool two = m != 0u;
+        const uint32_t j2 = two ? base + (uint32_t)__builtin_ctz(m) : j1;
+        if (two) {
+            m &= m - 1u;
+            if (!m) advance();
+        }
+        const auto a1 = ld(j1);
+        const auto a2 = ld(j2);
+        fn(a1, j1, true);
+        fn(a2, j2, two);
+    }
+    return true;
+}
+
+// Walk over the neighbour slots of lane i: with flat set, the flattened walk
+// when masks are in use and every lane's rows fit it, else walk_masks.  The
+// choice is wave-uniform and does not change the result.
+template <int BS, class Ld, class Fn>
+__device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
+                                          const RowMasks &mask, bool use, bool flat, uint32_t nslots, uint32_t i,
+                                          Ld &&ld, Fn &&fn) {
+    if (use && flat && walk_flat<BS>(tab, b, e, mask, nslots, i, ld, fn)) return;
+    walk_masks(b, e, mask, use, nslots, i, ld, fn);
 }
 
 // 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
@@ -311,41 +394,53 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     rows9(c, st, shift, pi, b, e);
     float S = 0.0f;
     // Branch-free candidate tests: a candidate outside h (or the particle
-    // itself) adds an exact zero to S.  The slot offset k runs in step across
-    // the wave, so the mask bit (1 << k) is a scalar; slots 32..63 of a long
-    // row go to the high words.
+    // itself, or past the end of the lane's row) adds an exact zero to S.
+    // The first 64 slots of every lane's row are scanned in wave-uniform
+    // loops up to the wave's longest row (slots 0..31, then 32..63): the slot
+    // offset k is a scalar, so a candidate's address is the lane's row offset
+    // plus a scalar (buffer soffset).  Each mask word is built as m = 2m +
+    // bit, one add-with-carry per candidate, which leaves slot k at bit
+    // n-1-k of the n slots scanned into it; one bit reversal per row moves it
+    // to bit k.  Slots past 64 (rows without masks) are tested per lane.
+    // Only the centre row (r = 4: dz = dy = 0) holds the particle itself.
+    const __amdgpu_buffer_rsrc_t pbuf = slot_buffer(Ps, nslots);
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t bb = b[r], len = e[r] - b[r];
-        const uint32_t kself = i - bb;  // >= len (wrapped) unless i is in this row
+        const uint32_t off = bb * (uint32_t)sizeof(float4);
+        const uint32_t kself = i - bb;  // < len only in the centre row
+        const uint32_t lw = wave_max<7>(len < 64u ? len : 64u);
+        const uint32_t lw0 = lw < 32u ? lw : 32u;
         uint32_t m0 = 0u, s0 = 0u, m1 = 0u, s1 = 0u;
-        const uint32_t l0 = len < 32u ? len : 32u;
-        for (uint32_t k = 0; k < l0; ++k) {
-            const float rr = dist2(pi, at32(Ps, bb + k));
-            const bool other = k != kself;
-            const bool in = rr <= c.h2 && other, ins = rr <= c.h2s && other;
+        auto test = [&](uint32_t k, uint32_t &m, uint32_t &sm) {
+            const float rr = dist2(pi, load_xyz(pbuf, off, k * (uint32_t)sizeof(float4)));
+            uint64_t live = lanes(k < len);
+            if (r == 4) live &= ~lanes(k == kself);
+            const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= c.h2s) & live;
+            const float t = keep(c.h2 - rr, in);
+            S += t * t * t;
+            m = shift_in(m, in);
+            sm = shift_in(sm, ins);
+        };
+        for (uint32_t k = 0; k < lw0; ++k) test(k, m0, s0);
+        for (uint32_t k = 32u; k < lw; ++k) test(k, m1, s1);
+        for (uint32_t k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
+            const float rr = dist2(pi, load_xyz(pbuf, off, k * (uint32_t)sizeof(float4)));
+            const bool in = rr <= c.h2 && (r != 4 || k != kself);
             const float t = in ? c.h2 - rr : 0.0f;
             S += t * t * t;
-            m0 |= (in ? 1u : 0u) << k;
-            s0 |= (ins ? 1u : 0u) << k;
-        }
-        for (uint32_t k = 32u; k < len; ++k) {  // rows of more than 32 slots
-            const float rr = dist2(pi, at32(Ps, bb + k));
-            const bool other = k != kself;
-            const bool in = rr <= c.h2 && other, ins = rr <= c.h2s && other;
-            const float t = in ? c.h2 - rr : 0.0f;
-            S += t * t * t;
-            const uint32_t bit = k < 64u ? 1u << (k & 31u) : 0u;
-            m1 |= in ? bit : 0u;
-            s1 |= ins ? bit : 0u;
         }
         const size_t at = (size_t)r * nslots + i;
+        const uint32_t sh0 = 32u - lw0;  // lw0 == 0 leaves m0 = s0 = 0
+        m0 = lw0 ? __brev(m0) >> sh0 : 0u;
+        s0 = lw0 ? __brev(s0) >> sh0 : 0u;
         if (len <= 32u) {
             M.w32[at] = m0;
             MS.w32[at] = s0;
         } else if (len <= 64u) {
-            M.w64[at] = (uint64_t)m1 << 32 | m0;
-            MS.w64[at] = (uint64_t)s1 << 32 | s0;
+            const uint32_t sh1 = 64u - lw;  // lw - 32 slots scanned into the high word (lw >= len > 32)
+            M.w64[at] = (uint64_t)(__brev(m1) >> sh1) << 32 | m0;
+            MS.w64[at] = (uint64_t)(__brev(s1) >> sh1) << 32 | s0;
         }
     }
     S *= c.mC6;  // sum of m * poly6
@@ -371,13 +466,13 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
 // 6. surface normals (computeNormals, fluidphysics.cu:154-210):
 // n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
 // gradPoly6's factor r.  Neighbours with density <= FLT_MIN (Q.w < 0) or at
-// r^2 <= FLT_MIN contribute zero.  normal_term is shared by k_normals and by
+// r^2 <= FLT_MIN (or passed with ok = false) contribute zero.  normal_term is shared by k_normals and by
 // k_forces, which computes the normals inside the force walk for scenes
 // without surface tension (own_normal); both give the same bits.
-__device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, float &sx, float &sy,
-                                            float &sz) {
+__device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, bool ok, float &sx,
+                                            float &sy, float &sz) {
     float t = fmaxf(c.h2 - rr, 0.0f);
-    t = rr > c.eps ? t : 0.0f;
+    t = ok && rr > c.eps ? t : 0.0f;
     const float w = t * t * fmaxf(q.w, 0.0f);
     sx += w * (pi.x - q.x);
     sy += w * (pi.y - q.y);
@@ -387,7 +482,7 @@ __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q
 template <int BS>
 __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Q,
-                                                const uint32_t *__restrict__ start, RowMasks M,
+                                                const uint32_t *__restrict__ start, RowMasks M, int flat,
                                                 float4 *__restrict__ N) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
@@ -405,9 +500,10 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
     uint32_t b[9], e[9];
     rows9(c, st, shift, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
-    walk_masks(
-        b, e, M, c.use_masks, nslots, i, [&](uint32_t j) { return at32(Q, j); },
-        [&](const float4 &q, uint32_t) { normal_term(c, pi, q, dist2(pi, q), sx, sy, sz); });
+    __shared__ uint2 tab[kFlatEntries * BS];
+    walk_rows<BS>(
+        tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
+        [&](const float4 &q, uint32_t, bool ok) { normal_term(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
 }
 
@@ -425,10 +521,10 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                                                const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                const float4 *__restrict__ Ls, const float4 *__restrict__ D,
                                                float4 *__restrict__ N, const uint32_t *__restrict__ start,
-                                               RowMasks M, float4 *__restrict__ Pout, float4 *__restrict__ Vout,
-                                               float4 *__restrict__ Aout, float4 *__restrict__ Lout,
-                                               float4 *__restrict__ X, float4 *__restrict__ Fdbg,
-                                               uint32_t *__restrict__ far_move) {
+                                               RowMasks M, int flat, float4 *__restrict__ Pout,
+                                               float4 *__restrict__ Vout, float4 *__restrict__ Aout,
+                                               float4 *__restrict__ Lout, float4 *__restrict__ X,
+                                               float4 *__restrict__ Fdbg, uint32_t *__restrict__ far_move) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
@@ -451,53 +547,81 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
         float nx = 0.f, ny = 0.f, nz = 0.f;  // own normal
         uint32_t b[9], e[9];
         rows9(c, st, shift, pi, b, e);
-        struct Cand {
-            float4 q;
-            float p;
-        };
-        walk_masks(
-            b, e, M, c.use_masks, nslots, i, [&](uint32_t j) { return Cand{at32(Q, j), at32(Pr, j)}; },
-            [&](const Cand &cj, uint32_t j) {
-                const float4 q = cj.q;
-                const float pj = cj.p;
-                const float rr = dist2(pi, q);
-                if (own_normal) normal_term(c, pi, q, rr, nx, ny, nz);
-                if (!(rr <= c.h2 && rr > c.eps)) return;
-                if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
-                const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
-                const float inv_r = frsq(rr);
-                const float r = rr * inv_r;
-                const float hr = fmaxf(c.h - r, 0.0f);
-                const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
-                const float irj = q.w;
-                const float mid = pj * iri2 + p_i * (irj * irj);
-                const float cpr = cm * mid;
-                Px += cpr * rx;
-                Py += cpr * ry;
-                Pz += cpr * rz;
-                if (c.has_visc) {
-                    const float4 vj = at32(Vs, j);
-                    const float4 dj = at32(D, j);
-                    const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
-                    if (ux < 0.0f) {
+        // One walk per combination of the scene's optional terms, so that a
+        // candidate's loads (V: velocity and density/sound speed; T: normal)
+        // are all issued with its position before any arithmetic.
+        __shared__ uint2 tab[kFlatEntries * BS];
+        auto walk = [&]<bool V, bool T>() {
+            struct Cand {
+                float4 q;
+                float p;
+                float4 v, d, n;
+            };
+            walk_rows<BS>(
+                tab, b, e, M, c.use_masks, flat != 0, nslots, i,
+                [&](uint32_t j) {
+                    Cand cd;
+                    cd.q = at32(Q, j);
+                    cd.p = at32(Pr, j);
+                    if constexpr (V) {
+                        cd.v = at32(Vs, j);
+                        cd.d = at32(D, j);
+                    }
+                    if constexpr (T) cd.n = at32(N, j);
+                    return cd;
+                },
+                [&](const Cand &cj, uint32_t, bool ok) {
+                    const float4 q = cj.q;
+                    const float pj = cj.p;
+                    const float rr = dist2(pi, q);
+                    if (own_normal) normal_term(c, pi, q, rr, ok, nx, ny, nz);
+                    if (!(ok && rr <= c.h2 && rr > c.eps)) return;
+                    if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
+                    const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
+                    const float inv_r = frsq(rr);
+                    const float r = rr * inv_r;
+                    const float hr = fmaxf(c.h - r, 0.0f);
+                    const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
+                    const float irj = q.w;
+                    const float mid = pj * iri2 + p_i * (irj * irj);
+                    const float cpr = cm * mid;
+                    Px += cpr * rx;
+                    Py += cpr * ry;
+                    Pz += cpr * rz;
+                    if constexpr (V) {
+                        // Approaching pairs only (ux < 0); the others add exact zeros.
+                        const float4 vj = cj.v, dj = cj.d;
+                        const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
                         const float mu = c.h * ux * frcp(rr + c.visc_eps);
                         const float Pi = -c.nu * (cs_i + dj.z) * mu * frcp(rho_i + dj.x);
-                        const float cv = cm * Pi;
+                        const float cv = keep(cm * Pi, lanes(ux < 0.0f));
                         Qx += cv * rx;
                         Qy += cv * ry;
                         Qz += cv * rz;
                     }
-                }
-                if (!own_normal && c.has_tension) {
-                    const float4 nj = at32(N, j);
-                    const float hr3r3 = hr * hr * hr * r * r * r;
-                    const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
-                    const float ch = -c.m2 * coh * inv_r;
-                    Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
-                    Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
-                    Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
-                }
-            });
+                    if constexpr (T) {
+                        const float4 nj = cj.n;
+                        const float hr3r3 = hr * hr * hr * r * r * r;
+                        const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
+                        const float ch = -c.m2 * coh * inv_r;
+                        Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
+                        Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
+                        Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
+                    }
+                });
+        };
+        const bool tension = !own_normal && c.has_tension;
+        if (c.has_visc) {
+            if (tension)
+                walk.template operator()<true, true>();
+            else
+                walk.template operator()<true, false>();
+        } else {
+            if (tension)
+                walk.template operator()<false, true>();
+            else
+                walk.template operator()<false, false>();
+        }
         if (own_normal) N[i] = make_float4(c.hmCg6 * nx, c.hmCg6 * ny, c.hmCg6 * nz, 0.0f);
         fP = make_float3(Px, Py, Pz);
         fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
@@ -592,7 +716,7 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                              uint32_t nslots, const float4 *__restrict__ Ps,
                                              const float4 *__restrict__ X, const float4 *__restrict__ Vh,
-                                             const uint32_t *__restrict__ start, RowMasks MS,
+                                             const uint32_t *__restrict__ start, RowMasks MS, int flat,
                                              const uint32_t *__restrict__ far_move, float4 *__restrict__ Vout) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
@@ -613,21 +737,23 @@ __global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, cons
     uint32_t b[9], e[9];
     rows9(c, st, shift, Ps[i], b, e);
     float sx = 0.f, sy = 0.f, sz = 0.f;
+    const float h2 = c.h2, eps = c.eps;
     // Both loads of a candidate are issued together; a candidate outside h
-    // (or with rho_i + rho_j <= FLT_MIN) adds exact zeros.
+    // (or with rho_i + rho_j <= FLT_MIN) adds exact zeros, branch-free.
     struct Cand {
         float4 x, v;
     };
-    walk_masks(
-        b, e, MS, masks_ok, nslots, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
-        [&](const Cand &cj, uint32_t) {
+    __shared__ uint2 tab[kFlatEntries * BS];
+    walk_rows<BS>(
+        tab, b, e, MS, masks_ok, flat != 0, nslots, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
+        [&](const Cand &cj, uint32_t, bool ok) {
             const float4 q = cj.x;
             const float4 vj = cj.v;
             const float rr = dist2(xi, q);
             const float sr = rho_i + q.w;
-            const bool in = rr <= c.h2 && sr > c.eps;
-            const float t = in ? c.h2 - rr : 0.0f;
-            const float w = in ? t * t * t * frcp(sr) : 0.0f;
+            const uint64_t in = lanes(rr <= h2) & lanes(sr > eps) & lanes(ok);
+            const float t = keep(h2 - rr, in);
+            const float w = keep(t * t * t * frcp(sr), in);
             sx += w * (vj.x - vi.x);
             sy += w * (vj.y - vi.y);
             sz += w * (vj.z - vi.z);
@@ -910,28 +1036,30 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
     });
 }
 
-void launch_normals(const Launch &L, const PoolView &pv, int bs, const float4 *Q, const uint32_t *start, RowMasks M,
-                    float4 *N) {
+void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Q, const uint32_t *start,
+                    RowMasks M, float4 *N) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, N);
+        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat ? 1 : 0,
+              N);
     });
 }
 
-void launch_forces(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *Q, const float *Pr,
+void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
+                   const float *Pr,
                    const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
                    const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
                    float4 *X, float4 *Fdbg, uint32_t *far_move) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
-              start, M, Pout, Vout, Aout, Lout, X, Fdbg, far_move);
+              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, far_move);
     });
 }
 
-void launch_xsph(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float4 *X, const float4 *Vh,
-                 const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout) {
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
+                 const float4 *Vh, const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, start, MS, far_move,
-              Vout);
+        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, start, MS,
+              flat ? 1 : 0, far_move, Vout);
     });
 }
 

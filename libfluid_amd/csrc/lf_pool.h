@@ -20,8 +20,12 @@
 namespace lf {
 
 // Workgroup sizes of the four neighbour passes (128, 256 or 512 each).
+// Launch shapes: workgroup size per neighbour pass, and whether the normals,
+// forces and XSPH passes walk the neighbour masks as one flattened stream
+// (1) or row by row (0).  Results do not depend on either.
 struct Tuning {
     int density = 256, normals = 256, forces = 256, xsph = 256;
+    int flat_normals = 1, flat_forces = 1, flat_xsph = 0;
 };
 
 struct SceneRec {

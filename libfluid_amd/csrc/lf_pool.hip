@@ -27,7 +27,18 @@ namespace lf {
 namespace {
 
 constexpr double kPi = 3.141592654;  // CUDART_PI_F as defined by the reference (fluiddefines.h:16)
-constexpr double kSkin = 0.1;        // XSPH skin radius (1 + kSkin) h
+
+// XSPH skin radius (1 + skin) h.  The integrator moves a particle by
+// v dt + a dt^2 / 2 with |a| <= acc_limit and |v| at most about vel_limit
+// (velocities are clamped to it before XSPH blends them), so the skin is
+// sized for both particles of a pair moving that far, with 10% to spare,
+// between 0.02 and 0.1.  A larger move sets the scene's far_move flag and
+// the XSPH pass scans (the result is the same; only the speed differs).
+double skin_of(const lf_params &p, double dt) {
+    const double move = (double)p.vel_limit * dt + 0.5 * (double)p.acc_limit * dt * dt;
+    const double s = 2.2 * move / (double)p.h;
+    return (s > 0.0 && s < 0.1) ? (s > 0.02 ? s : 0.02) : 0.1;
+}
 
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
@@ -59,7 +70,8 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.h2 = p.h * p.h;  // FP32 product, the pair-predicate threshold (m_h2)
     c.inv_h = (float)(1.0 / h);
     c.eps = FLT_MIN;
-    c.h2s = (float)((1.0 + kSkin) * (1.0 + kSkin) * (double)c.h2);
+    const double skin = skin_of(p, dt);
+    c.h2s = (float)((1.0 + skin) * (1.0 + skin) * (double)c.h2);
     // A pair within h after the move (FP32 predicate: true distance below
     // h (1 + 1e-6)) was within h (1 + 1e-6) + 2 d before it, which the FP32
     // skin predicate accepts when that squared, times (1 + 1e-6), is <= h2s.
@@ -481,13 +493,16 @@ void Pool::enqueue_substep() {
     timed(LF_K_DENSITY,
           [&](const Launch &L) { launch_density(L, pv, tune_.density, P1_, start_, D_, Q_, Pr_, M_, MS_, far_move_); });
     if (any_separate_normals())
-        timed(LF_K_NORMALS, [&](const Launch &L) { launch_normals(L, pv, tune_.normals, Q_, start_, M_, N_); });
+        timed(LF_K_NORMALS, [&](const Launch &L) {
+            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, Q_, start_, M_, N_);
+        });
     timed(LF_K_FORCES, [&](const Launch &L) {
-        launch_forces(L, pv, tune_.forces, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_, P0_, V2_, A0_, L0_, X_,
-                      (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, far_move_);
+        launch_forces(L, pv, tune_.forces, tune_.flat_forces != 0, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
+                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, far_move_);
     });
-    timed(LF_K_XSPH,
-          [&](const Launch &L) { launch_xsph(L, pv, tune_.xsph, P1_, X_, V2_, start_, MS_, far_move_, V0_); });
+    timed(LF_K_XSPH, [&](const Launch &L) {
+        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph != 0, P1_, X_, V2_, start_, MS_, far_move_, V0_);
+    });
 }
 
 // Launches of the executable graph may still be queued: wait for them before
@@ -843,15 +858,19 @@ int Pool::set_tuning(const Tuning &t) {
     if (!valid_block_size(t.density) || !valid_block_size(t.normals) || !valid_block_size(t.forces) ||
         !valid_block_size(t.xsph))
         return fail(LF_ERR_INVALID, "tuning: block sizes must be 128, 256 or 512");
+    if ((t.flat_normals | t.flat_forces | t.flat_xsph) & ~1)
+        return fail(LF_ERR_INVALID, "tuning: walk modes must be 0 (row by row) or 1 (flattened)");
     tune_ = t;
     drop_graph();
     return LF_OK;
 }
 
-// Each candidate block size runs `reps` substeps from the same saved state,
-// all four neighbour passes timed with events; the state (and every scene's
-// step count and order flag) is restored afterwards, so tuning leaves the
-// trajectories untouched.
+// Each candidate launch shape runs `reps` substeps from the same saved
+// state, all four neighbour passes timed with events: first the three block
+// sizes (walk modes as set), then, at the fastest block sizes, both walk
+// modes of the normals, forces and XSPH passes.  The state (and every
+// scene's step count and order flag) is restored afterwards, so tuning
+// leaves the trajectories untouched.
 int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     if (reps == 0) return fail(LF_ERR_INVALID, "autotune: zero reps");
     bool any = false;
@@ -871,26 +890,57 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     keep_live.swap(prof_live_);  // the caller's pending timings stay untouched
     prof_mask_ = 0;
     const int kinds[4] = {LF_K_DENSITY, LF_K_NORMALS, LF_K_FORCES, LF_K_XSPH};
-    const int sizes[3] = {128, 256, 512};
-    double ms[4][3] = {};
     int rc = LF_OK;
-    for (int b = 0; b < 3 && !rc; ++b) {
-        tune_ = Tuning{sizes[b], sizes[b], sizes[b], sizes[b]};
+    // Average ms per launch of each neighbour pass under launch shape t.
+    auto time_shape = [&](const Tuning &t, double out[4]) {
+        tune_ = t;
         for (int k = 0; k < 5 && !rc; ++k) {
             hipError_t e = hipMemcpyAsync(state[k], save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
             if (e != hipSuccess) rc = fail(LF_ERR_HIP, "autotune: %s", hipGetErrorString(e));
         }
-        if (rc) break;
+        if (rc) return;
         enqueue_substep();  // warm-up, untimed
         for (int k = 0; k < 4; ++k) prof_mask_ |= 1u << kinds[k];
         for (uint32_t r = 0; r < reps; ++r) enqueue_substep();
         prof_mask_ = 0;
-        if ((rc = check_launch("autotune"))) break;
+        if ((rc = check_launch("autotune"))) return;
         uint64_t cnt[LF_K_COUNT];
         double tot[LF_K_COUNT];
-        if ((rc = profile_read(cnt, tot))) break;
-        for (int k = 0; k < 4; ++k) ms[k][b] = cnt[kinds[k]] ? tot[kinds[k]] / (double)cnt[kinds[k]] : 0.0;
+        if ((rc = profile_read(cnt, tot))) return;
+        for (int k = 0; k < 4; ++k) out[k] = cnt[kinds[k]] ? tot[kinds[k]] / (double)cnt[kinds[k]] : 0.0;
+    };
+    const int sizes[3] = {128, 256, 512};
+    double ms[4][3] = {};
+    for (int b = 0; b < 3 && !rc; ++b) {
+        Tuning t = before;
+        t.density = t.normals = t.forces = t.xsph = sizes[b];
+        double out[4] = {};
+        time_shape(t, out);
+        for (int k = 0; k < 4; ++k) ms[k][b] = out[k];
     }
+    int best[4];
+    for (int k = 0; k < 4; ++k) {
+        best[k] = 1;
+        for (int b = 0; b < 3; ++b)
+            if (ms[k][b] > 0.0 && ms[k][b] < ms[k][best[k]]) best[k] = b;
+    }
+    Tuning chosen = before;
+    chosen.density = sizes[best[0]];
+    chosen.normals = sizes[best[1]];
+    chosen.forces = sizes[best[2]];
+    chosen.xsph = sizes[best[3]];
+    double walk[2][4] = {};  // [row by row, flattened][pass]
+    for (int f = 0; f < 2 && !rc; ++f) {
+        Tuning t = chosen;
+        t.flat_normals = t.flat_forces = t.flat_xsph = f;
+        time_shape(t, walk[f]);
+    }
+    auto pick = [&](int k) {  // flattened unless row by row was measurably faster
+        return (walk[0][k] > 0.0 && walk[0][k] < walk[1][k]) ? 0 : 1;
+    };
+    chosen.flat_normals = pick(1);
+    chosen.flat_forces = pick(2);
+    chosen.flat_xsph = pick(3);
     prof_mask_ = keep_mask;
     for (auto &r : prof_live_) {  // left over after an error
         ev_pool_.push_back(r.a);
@@ -908,13 +958,7 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
         drop_graph();
         return rc;
     }
-    int best[4];
-    for (int k = 0; k < 4; ++k) {
-        best[k] = 1;
-        for (int b = 0; b < 3; ++b)
-            if (ms[k][b] > 0.0 && ms[k][b] < ms[k][best[k]]) best[k] = b;
-    }
-    tune_ = Tuning{sizes[best[0]], sizes[best[1]], sizes[best[2]], sizes[best[3]]};
+    tune_ = chosen;
     drop_graph();
     if (ms_out)
         for (int k = 0; k < 4; ++k)

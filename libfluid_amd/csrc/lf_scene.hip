@@ -29,10 +29,28 @@ namespace {
 
 uint64_t cells_of(const lf_params *p) { return (uint64_t)p->grid[0] * p->grid[1] * p->grid[2]; }
 
+// Block sizes replace those of t; its walk modes stay.
 int tuning_from(const int bs[4], lf::Tuning *t) {
     if (!bs) return fail(LF_ERR_INVALID, "null block sizes");
-    *t = lf::Tuning{bs[0], bs[1], bs[2], bs[3]};
+    t->density = bs[0];
+    t->normals = bs[1];
+    t->forces = bs[2];
+    t->xsph = bs[3];
     return LF_OK;
+}
+
+int walks_from(const int flat[3], lf::Tuning *t) {
+    if (!flat) return fail(LF_ERR_INVALID, "null walk modes");
+    t->flat_normals = flat[0];
+    t->flat_forces = flat[1];
+    t->flat_xsph = flat[2];
+    return LF_OK;
+}
+
+void walks_to(const lf::Tuning &t, int flat[3]) {
+    flat[0] = t.flat_normals;
+    flat[1] = t.flat_forces;
+    flat[2] = t.flat_xsph;
 }
 
 void tuning_to(const lf::Tuning &t, int bs[4]) {
@@ -165,9 +183,22 @@ int lf_copy_probe(lf_scene *s, size_t bytes, int repeats, double *ms) {
 
 int lf_set_tuning(lf_scene *s, const int block_sizes[4]) {
     if (!s) return fail(LF_ERR_INVALID, "null scene");
-    lf::Tuning t;
+    lf::Tuning t = s->pool.tuning();
     if (int rc = tuning_from(block_sizes, &t)) return rc;
     return s->pool.set_tuning(t);
+}
+
+int lf_set_walks(lf_scene *s, const int flat[3]) {
+    if (!s) return fail(LF_ERR_INVALID, "null scene");
+    lf::Tuning t = s->pool.tuning();
+    if (int rc = walks_from(flat, &t)) return rc;
+    return s->pool.set_tuning(t);
+}
+
+int lf_get_walks(const lf_scene *s, int flat[3]) {
+    if (!s || !flat) return fail(LF_ERR_INVALID, "null argument");
+    walks_to(s->pool.tuning(), flat);
+    return LF_OK;
 }
 
 int lf_get_tuning(const lf_scene *s, int block_sizes[4]) {
@@ -342,9 +373,22 @@ int lf_batch_usage(const lf_batch *b, uint32_t *scenes, uint32_t *slots_used, ui
 
 int lf_batch_set_tuning(lf_batch *b, const int block_sizes[4]) {
     if (!b) return fail(LF_ERR_INVALID, "null batch");
-    lf::Tuning t;
+    lf::Tuning t = b->pool.tuning();
     if (int rc = tuning_from(block_sizes, &t)) return rc;
     return b->pool.set_tuning(t);
+}
+
+int lf_batch_set_walks(lf_batch *b, const int flat[3]) {
+    if (!b) return fail(LF_ERR_INVALID, "null batch");
+    lf::Tuning t = b->pool.tuning();
+    if (int rc = walks_from(flat, &t)) return rc;
+    return b->pool.set_tuning(t);
+}
+
+int lf_batch_get_walks(const lf_batch *b, int flat[3]) {
+    if (!b || !flat) return fail(LF_ERR_INVALID, "null argument");
+    walks_to(b->pool.tuning(), flat);
+    return LF_OK;
 }
 
 int lf_batch_get_tuning(const lf_batch *b, int block_sizes[4]) {

@@ -206,9 +206,21 @@ class Scene:
         check(lib.lf_get_tuning(self._h, bs), "lf_get_tuning")
         return tuple(bs)
 
+    def set_walks(self, flat) -> None:
+        """Walk modes of the normals, forces and XSPH passes (1 flattened,
+        0 row by row)."""
+        f = (C.c_int * 3)(*[int(x) for x in flat])
+        check(lib.lf_set_walks(self._h, f), "lf_set_walks")
+
+    def walks(self) -> tuple:
+        f = (C.c_int * 3)()
+        check(lib.lf_get_walks(self._h, f), "lf_get_walks")
+        return tuple(f)
+
     def autotune(self, reps: int = 5) -> np.ndarray:
-        """Keeps the fastest block size per pass; returns the (4, 3) ms table
-        (passes density/normals/forces/xsph x block sizes 128/256/512)."""
+        """Keeps the fastest block size per pass, then the faster walk mode per
+        pass; returns the (4, 3) ms table of the block-size sweep (passes
+        density/normals/forces/xsph x block sizes 128/256/512)."""
         ms = (C.c_double * 12)()
         check(lib.lf_autotune(self._h, int(reps), ms), "lf_autotune")
         return np.array(ms[:], dtype=np.float64).reshape(4, 3)
@@ -347,6 +359,15 @@ class Batch:
         bs = (C.c_int * 4)()
         check(lib.lf_batch_get_tuning(self._h, bs), "lf_batch_get_tuning")
         return tuple(bs)
+
+    def set_walks(self, flat) -> None:
+        f = (C.c_int * 3)(*[int(x) for x in flat])
+        check(lib.lf_batch_set_walks(self._h, f), "lf_batch_set_walks")
+
+    def walks(self) -> tuple:
+        f = (C.c_int * 3)()
+        check(lib.lf_batch_get_walks(self._h, f), "lf_batch_get_walks")
+        return tuple(f)
 
     def autotune(self, reps: int = 5) -> np.ndarray:
         ms = (C.c_double * 12)()

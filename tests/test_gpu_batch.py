@@ -135,7 +135,7 @@ def test_batch_full_reports_full(gpu):
 def test_autotune_and_block_sizes_do_not_change_results(gpu):
     cfgs = mixed_scenes(3, seed=9)
     outs = []
-    for mode in ("default", "autotune", "small", "large"):
+    for mode in ("default", "autotune", "small", "large", "rows"):
         with batch_for(gpu, cfgs) as b:
             ids = [b.add(c.params, c.n, c.dt) for c in cfgs]
             for i, c in zip(ids, cfgs):
@@ -145,6 +145,10 @@ def test_autotune_and_block_sizes_do_not_change_results(gpu):
                 ms = b.autotune(2)
                 assert ms.shape == (4, 3) and np.all(ms[[0, 2, 3]] > 0)
                 assert all(x in (128, 256, 512) for x in b.tuning())
+                assert all(f in (0, 1) for f in b.walks())
+            elif mode == "rows":
+                b.set_walks((0, 0, 0))
+                assert b.walks() == (0, 0, 0)
             elif mode == "small":
                 b.set_tuning((128, 128, 128, 128))
             elif mode == "large":
@@ -152,7 +156,7 @@ def test_autotune_and_block_sizes_do_not_change_results(gpu):
             b.step(8)
             b.sync()
             outs.append([b.download(i) for i in ids])
-    for m in range(1, 4):
+    for m in range(1, 5):
         for j in range(len(cfgs)):
             assert_same(outs[0][j], outs[m][j], f"mode {m} scene {j}")
 

@@ -248,29 +248,34 @@ def test_full_size_column_c2(gpu, oracle_omp):
     compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c2 replay from step 5")
 
 
-@pytest.mark.parametrize("case", ["c1", "double_dam", "dense"])
+@pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed"])
 def test_neighbour_masks_are_bitwise_transparent(gpu, case):
-    """Walking the density pass's neighbour masks gives bitwise the same
-    trajectory as testing every stencil slot (same neighbours, same order).
-    c1 moves little per substep (XSPH walks the skin masks); the double dam's
-    dt = 2 ms moves particles farther than the skin allows (XSPH scans); the
-    dense block has stencil rows longer than 64 slots (those rows are scanned
-    in every pass)."""
+    """Walking the density pass's neighbour masks, as one flattened stream or
+    row by row, gives bitwise the same trajectory as testing every stencil
+    slot (same neighbours, same order).  c1 moves little per substep (XSPH
+    walks the skin masks); the double dam's dt = 2 ms moves particles farther
+    than the skin allows (XSPH scans); the dense block has stencil rows longer
+    than 64 slots (those rows are scanned in every pass, and their waves walk
+    row by row); the packed block (about 12 particles a cell) has rows of 33
+    to 64 slots, which the flattened walk splits into two 32-slot parts."""
     cfg = {"c1": configs.dambreak_c1, "double_dam": lambda: configs.double_dam(n=4096, res=16),
-           "dense": configs.dense_block}[case]()
+           "dense": configs.dense_block,
+           "packed": lambda: configs.dense_block(24, 24, 24, spacing=configs.H / 2.3)}[case]()
     outs = []
-    for scan in (False, True):
+    for scan, flat in ((True, 1), (False, 1), (False, 0)):
         sc = gpu.Scene(cfg.params, cfg.n)
         try:
             sc.upload(cfg.pos, cfg.vel)
             sc.set_debug(forces=False, scan=scan)
+            sc.set_walks((flat, flat, flat))
             sc.run(cfg.dt, 20)
             sc.sync()
             outs.append(sc.download(L.LF_ORDER_ID))
         finally:
             sc.close()
-    for k in ("pos", "vel", "acc", "colour"):
-        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs between mask walks and full scans"
+    for m, name in ((1, "flattened mask walks"), (2, "row-by-row mask walks")):
+        for k in ("pos", "vel", "acc", "colour"):
+            assert np.array_equal(outs[0][k], outs[m][k]), f"{case}: {k} differs between {name} and full scans"
 
 
 def test_normals_in_force_walk_match_separate_pass(gpu):
