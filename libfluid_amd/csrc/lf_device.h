@@ -160,23 +160,22 @@ __device__ __forceinline__ T at32(const T *__restrict__ base, uint32_t j) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + j * (uint32_t)sizeof(T));
 }
 
-// Raw buffer view of a particle array of n float4 slots (range-checked: a
-// load at or past the end returns zeros).  Loads through it take a per-lane
-// byte offset and a wave-uniform one (soffset), so a candidate loop whose
-// slot offset runs in step across the wave advances its address with scalar
-// adds.  0x00020000 is the gfx9 raw-buffer descriptor word 3 (32-bit data
-// format, no swizzle).
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_buffer(const float4 *base, uint32_t n) {
-    const uint64_t bytes = (uint64_t)n * sizeof(float4);
-    const int records = bytes >= 0xFFFFFFFFull ? (int)0xFFFFFFFFu : (int)(uint32_t)bytes;
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4 *>(base), (short)0, records, 0x00020000);
+// Raw buffer views (range-checked: a load at or past the end returns zeros).
+// Loads through them take a per-lane byte offset and a wave-uniform one
+// (soffset), so a candidate loop whose slot offset runs in step across the
+// wave advances its address with scalar adds.  0x00020000 is the gfx9
+// raw-buffer descriptor word 3 (32-bit data format, no swizzle).
+// A plane of n floats.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_buffer(const float *base, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(n * (uint32_t)sizeof(float)),
+                                             0x00020000);
 }
-// x, y, z of the slot at byte offset voff + soff (soff wave-uniform).
-__device__ __forceinline__ float4 load_xyz(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, (int)voff, (int)soff, 0);
-    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
+// Four consecutive floats at byte offset voff + soff (soff wave-uniform;
+// dword alignment suffices for buffer loads).
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
 }
-
 // Largest v (v < 2^BITS) over the wave's active lanes, by BITS ballots.
 template <int BITS>
 __device__ __forceinline__ uint32_t wave_max(uint32_t v) {

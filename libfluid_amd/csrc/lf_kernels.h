@@ -36,9 +36,10 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err);
-void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const uint32_t *start, float4 *D,
-                    float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move);
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *err);
+void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
+                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
+                    uint32_t *far_move);
 // The neighbour passes take `flat`: walk the masks as one flattened stream
 // of set bits (where the rows fit) rather than row by row; results are the
 // same either way.  Only scenes without own_normal are processed by

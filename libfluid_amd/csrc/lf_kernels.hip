@@ -194,7 +194,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
                                                    const float4 *__restrict__ V, const float4 *__restrict__ A,
                                                    const float4 *__restrict__ L, float4 *__restrict__ Ps,
                                                    float4 *__restrict__ Vs, float4 *__restrict__ As,
-                                                   float4 *__restrict__ Ls, uint32_t *__restrict__ err) {
+                                                   float4 *__restrict__ Ls, float *__restrict__ soa, uint32_t nslots,
+                                                   uint32_t *__restrict__ err) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
@@ -213,7 +214,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
         f = s;  // bucket too large to rank: keep the provisional slot, report it
         if (s == lo) atomicOr(err, 1u);
     }
-    Ps[f] = P[src];
+    const float4 p = P[src];
+    Ps[f] = p;
+    soa[f] = p.x;
+    soa[nslots + f] = p.y;
+    soa[2 * (size_t)nslots + f] = p.z;
     Vs[f] = V[src];
     As[f] = A[src];
     Ls[f] = L[src];
@@ -373,7 +378,8 @@ __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_
 template <int BS>
 __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Ps,
-                                                const uint32_t *__restrict__ start, float4 *__restrict__ D,
+                                                const float *__restrict__ soa, const uint32_t *__restrict__ start,
+                                                float4 *__restrict__ D,
                                                 float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
                                                 RowMasks MS, uint32_t *__restrict__ far_move) {
     uint32_t slot0;
@@ -396,24 +402,27 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     // Branch-free candidate tests: a candidate outside h (or the particle
     // itself, or past the end of the lane's row) adds an exact zero to S.
     // The first 64 slots of every lane's row are scanned in wave-uniform
-    // loops up to the wave's longest row (slots 0..31, then 32..63): the slot
-    // offset k is a scalar, so a candidate's address is the lane's row offset
+    // loops up to the wave's longest row (slots 0..31, then 32..63), four
+    // slots per step: the sorted positions are also kept as x, y and z
+    // planes, so one 16-byte load per plane brings four candidates.  The
+    // slot offset k is a scalar, so a load's address is the lane's row offset
     // plus a scalar (buffer soffset).  Each mask word is built as m = 2m +
     // bit, one add-with-carry per candidate, which leaves slot k at bit
     // n-1-k of the n slots scanned into it; one bit reversal per row moves it
     // to bit k.  Slots past 64 (rows without masks) are tested per lane.
     // Only the centre row (r = 4: dz = dy = 0) holds the particle itself.
-    const __amdgpu_buffer_rsrc_t pbuf = slot_buffer(Ps, nslots);
+    const __amdgpu_buffer_rsrc_t px = plane_buffer(soa, nslots), py = plane_buffer(soa + nslots, nslots),
+                                 pz = plane_buffer(soa + 2 * (size_t)nslots, nslots);
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t bb = b[r], len = e[r] - b[r];
-        const uint32_t off = bb * (uint32_t)sizeof(float4);
+        const uint32_t off = bb * (uint32_t)sizeof(float);
         const uint32_t kself = i - bb;  // < len only in the centre row
         const uint32_t lw = wave_max<7>(len < 64u ? len : 64u);
         const uint32_t lw0 = lw < 32u ? lw : 32u;
         uint32_t m0 = 0u, s0 = 0u, m1 = 0u, s1 = 0u;
-        auto test = [&](uint32_t k, uint32_t &m, uint32_t &sm) {
-            const float rr = dist2(pi, load_xyz(pbuf, off, k * (uint32_t)sizeof(float4)));
+        auto test = [&](uint32_t k, float qx, float qy, float qz, uint32_t &m, uint32_t &sm) {
+            const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
             uint64_t live = lanes(k < len);
             if (r == 4) live &= ~lanes(k == kself);
             const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= c.h2s) & live;
@@ -422,10 +431,19 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             m = shift_in(m, in);
             sm = shift_in(sm, ins);
         };
-        for (uint32_t k = 0; k < lw0; ++k) test(k, m0, s0);
-        for (uint32_t k = 32u; k < lw; ++k) test(k, m1, s1);
+        // Slots k0 .. min(k0 + 4, kend) - 1 (kend wave-uniform).
+        auto four = [&](uint32_t k0, uint32_t kend, uint32_t &m, uint32_t &sm) {
+            const uint32_t so = k0 * (uint32_t)sizeof(float);
+            const float4 X = load4(px, off, so), Y = load4(py, off, so), Z = load4(pz, off, so);
+            test(k0, X.x, Y.x, Z.x, m, sm);
+            if (k0 + 1u < kend) test(k0 + 1u, X.y, Y.y, Z.y, m, sm);
+            if (k0 + 2u < kend) test(k0 + 2u, X.z, Y.z, Z.z, m, sm);
+            if (k0 + 3u < kend) test(k0 + 3u, X.w, Y.w, Z.w, m, sm);
+        };
+        for (uint32_t k = 0; k < lw0; k += 4u) four(k, lw0, m0, s0);
+        for (uint32_t k = 32u; k < lw; k += 4u) four(k, lw, m1, s1);
         for (uint32_t k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
-            const float rr = dist2(pi, load_xyz(pbuf, off, k * (uint32_t)sizeof(float4)));
+            const float rr = dist2(pi, at32(Ps, bb + k));
             const bool in = rr <= c.h2 && (r != 4 || k != kself);
             const float t = in ? c.h2 - rr : 0.0f;
             S += t * t * t;
@@ -1023,16 +1041,17 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, uint32_t *err) {
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *err) {
     LF_GO(L, k_gather, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          A, Lw, Ps, Vs, As, Ls, err);
+          A, Lw, Ps, Vs, As, Ls, soa, pv.nslots, err);
 }
 
-void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const uint32_t *start, float4 *D,
-                    float4 *Q, float *Pr, RowMasks M, RowMasks MS, uint32_t *far_move) {
+void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
+                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
+                    uint32_t *far_move) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, start, D, Q, Pr, M, MS,
-              far_move);
+        LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
+              MS, far_move);
     });
 }
 

@@ -125,7 +125,7 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
 void Pool::release() {
     stop_exporter();
     drop_graph();
-    void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, M_.w32, M_.w64, MS_.w32,
+    void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, soa_, M_.w32, M_.w64, MS_.w32,
                     MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, far_move_, d_cs_,
                     d_chunk_};
     for (void *b : bufs)
@@ -176,6 +176,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     for (float4 **b : f4)
         if (!rc) rc = dalloc(b, n);
     if (!rc) rc = dalloc(&Pr_, n);
+    if (!rc) rc = dalloc(&soa_, 3 * n);
     if (!rc) rc = dalloc(&M_.w32, 9 * n);
     if (!rc) rc = dalloc(&M_.w64, 9 * n);
     if (!rc) rc = dalloc(&MS_.w32, 9 * n);
@@ -480,7 +481,7 @@ void Pool::enqueue_sort() {
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, nbuckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
-        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, L0_, P1_, V1_, A1_, L1_, err_);
+        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, L0_, P1_, V1_, A1_, L1_, soa_, err_);
     });
 }
 
@@ -491,7 +492,7 @@ void Pool::enqueue_substep() {
     const PoolView pv = view();
     enqueue_sort();
     timed(LF_K_DENSITY,
-          [&](const Launch &L) { launch_density(L, pv, tune_.density, P1_, start_, D_, Q_, Pr_, M_, MS_, far_move_); });
+          [&](const Launch &L) { launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, far_move_); });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
             launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, Q_, start_, M_, N_);
