@@ -12,7 +12,7 @@ constexpr uint32_t kChunk = 512;              // particle-slot granularity of sc
 constexpr uint32_t kNoScene = 0xFFFFFFFFu;    // chunk owned by no scene
 constexpr uint32_t kMaxParticles = 1u << 28;  // pool slots: 16-byte records addressed with 32-bit offsets
 constexpr uint32_t kRankFixMax = 4096;        // largest bucket the in-cell id ranking handles
-constexpr int kScanPerThread = 16;            // cell-count scan: entries per thread
+constexpr int kScanPerThread = 4;             // cell-count scan: entries per thread (one 16-byte load)
 constexpr int kScanTile = kBlock * kScanPerThread;
 
 // Neighbour masks of one radius for 9 stencil rows x nslots particle slots,

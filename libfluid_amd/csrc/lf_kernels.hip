@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, 
 
 // ---------------------------------------------------------------------------
 // 2. cell start table: exclusive scan of the bucket counts
-// (thrust::exclusive_scan, fluidsystem.cu:316), three phases over all buckets
+// (thrust::exclusive_scan, fluidsystem.cu:316), two phases over all buckets
 // of the pool.
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
@@ -104,59 +104,53 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t *sh, ui
     return off + incl - v;
 }
 
+// The counts of one thread: kScanPerThread (= 4) consecutive buckets from
+// `base`, one 16-byte load (tiles start at multiples of kScanTile, so base is
+// 16-byte aligned) except in the last, partial tile.
+__device__ __forceinline__ uint4 scan_load(const uint32_t *__restrict__ counts, uint32_t base, uint32_t m) {
+    if (base + 4u <= m) return *reinterpret_cast<const uint4 *>(counts + base);
+    return make_uint4(base < m ? counts[base] : 0u, base + 1u < m ? counts[base + 1u] : 0u,
+                      base + 2u < m ? counts[base + 2u] : 0u, base + 3u < m ? counts[base + 3u] : 0u);
+}
+
+// Phase 1: the sum of each tile of kScanTile buckets.
 __global__ __launch_bounds__(kBlock) void k_scan_sum(const uint32_t *__restrict__ counts, uint32_t m,
                                                      uint32_t *__restrict__ bsum) {
     __shared__ uint32_t sh[kBlock / 64];
-    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) s += (base + k < m) ? counts[base + k] : 0u;
+    const uint4 v = scan_load(counts, blockIdx.x * kScanTile + threadIdx.x * kScanPerThread, m);
     uint32_t tot;
-    (void)block_excl_scan(s, sh, tot);
+    (void)block_excl_scan(v.x + v.y + v.z + v.w, sh, tot);
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// One workgroup: exclusive scan of the per-tile sums; writes the grand total
-// as the end sentinel start[m].
-__global__ __launch_bounds__(kBlock) void k_scan_top(uint32_t *__restrict__ bsum, uint32_t nb,
-                                                     uint32_t *__restrict__ start, uint32_t m) {
-    __shared__ uint32_t sh[kBlock / 64];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nb; base += kBlock) {
-        const uint32_t idx = base + threadIdx.x;
-        const uint32_t v = idx < nb ? bsum[idx] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, sh, tot);
-        if (idx < nb) bsum[idx] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0) start[m] = carry;
-}
-
-// Local scan of each tile plus its offset; clears the counts for the next
-// substep.
+// Phase 2: each tile adds up the sums of the tiles before it (at most a few
+// thousand, cached), scans its own buckets, writes their starts and clears
+// their counts for the next substep; the last tile writes the end sentinel
+// start[m].
 __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ counts, uint32_t m,
-                                                      const uint32_t *__restrict__ bsum,
+                                                      const uint32_t *__restrict__ bsum, uint32_t nb,
                                                       uint32_t *__restrict__ start) {
     __shared__ uint32_t sh[kBlock / 64];
+    uint32_t before = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) before += bsum[t];
+    uint32_t prefix;
+    (void)block_excl_scan(before, sh, prefix);
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
-    uint32_t v[kScanPerThread];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) {
-        v[k] = (base + k < m) ? counts[base + k] : 0u;
-        s += v[k];
-    }
+    const uint4 v = scan_load(counts, base, m);
     uint32_t tot;
-    uint32_t run = bsum[blockIdx.x] + block_excl_scan(s, sh, tot);
-#pragma unroll
-    for (int k = 0; k < kScanPerThread; ++k) {
-        if (base + k < m) {
-            start[base + k] = run;
+    const uint32_t run = prefix + block_excl_scan(v.x + v.y + v.z + v.w, sh, tot);
+    const uint4 st = make_uint4(run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z);
+    if (base + 4u <= m) {
+        *reinterpret_cast<uint4 *>(start + base) = st;
+        *reinterpret_cast<uint4 *>(counts + base) = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+        const uint32_t sv[4] = {st.x, st.y, st.z, st.w};
+        for (uint32_t k = 0; k < 4u && base + k < m; ++k) {
+            start[base + k] = sv[k];
             counts[base + k] = 0u;
         }
-        run += v[k];
     }
+    if (blockIdx.x == nb - 1u && threadIdx.x == 0) start[m] = prefix + tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,8 +1023,7 @@ uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
     const uint32_t nb = scan_tiles(m);
     LF_GO(L, k_scan_sum, nb, kBlock, L.a, nullptr, (const uint32_t *)counts, m, bsum);
-    LF_GO(L, k_scan_top, 1, kBlock, nullptr, nullptr, bsum, nb, start, m);
-    LF_GO(L, k_scan_down, nb, kBlock, nullptr, L.b, counts, m, (const uint32_t *)bsum, start);
+    LF_GO(L, k_scan_down, nb, kBlock, nullptr, L.b, counts, m, (const uint32_t *)bsum, nb, start);
 }
 
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
