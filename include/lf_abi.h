@@ -126,7 +126,9 @@ enum lf_field {
     LF_FIELD_START = 1,       /* uint32[ncells+2]: cell start table; [ncells] = first out-of-grid slot */
     LF_FIELD_SORTED_IDS = 2,  /* uint32[n]: particle id in each slot of the new cell order */
     LF_FIELD_DENSITY = 3,     /* float4[n]: (density, pressure, sound speed, 1/density), cell order */
-    LF_FIELD_NORMAL = 4,      /* float4[n]: (normal xyz, 0), cell order */
+    LF_FIELD_NORMAL = 4,      /* float4[n]: (normal xyz, 0), cell order; computed when read
+                                 (from the last substep's data) for scenes without surface
+                                 tension, which skip them while stepping */
     LF_FIELD_FORCES = 5,      /* float4[4n]: pressure, viscosity, tension, adhesion per particle
                                  (written only with LF_DEBUG_FORCES), cell order */
     LF_FIELD_VEL_HALF = 6,    /* float4[n]: velocity after integration, before XSPH, cell order */
@@ -138,8 +140,9 @@ int lf_get_field(lf_scene *s, int field, void *dst, size_t bytes);
  * (LF_FIELD_FORCES) each substep; LF_DEBUG_SCAN makes the normals, forces and
  * XSPH passes test every slot of the 27-cell stencil instead of walking the
  * neighbour masks the density pass records; LF_DEBUG_NORMALS_PASS runs the
- * normals as their own pass also when surface tension is zero (otherwise the
- * force pass computes them).  Results are bitwise the same either way. */
+ * normals pass every substep also when surface tension is zero (otherwise
+ * the normals, which feed only the tension force, are computed when read).
+ * Results are bitwise the same either way. */
 #define LF_DEBUG_FORCES 1
 #define LF_DEBUG_SCAN 2
 #define LF_DEBUG_NORMALS_PASS 4

@@ -51,10 +51,10 @@ struct Consts {
     // density + EOS
     float mC6;  // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
-    // normals: own_normal = the force pass computes this scene's normals
-    // (no surface tension, so no force term reads a neighbour's normal)
+    // normals: lazy_normal = no force term reads this scene's normals (no
+    // surface tension), so they are computed only when read
     float hmCg6;  // h * m * (-945/(32 pi h^9))
-    int own_normal;
+    int lazy_normal;
     // forces
     float mCgs;  // m * (-45/(pi h^6))
     float nu, sigma, alpha, m2, sigma_m, visc_eps, halfh, coh_off;

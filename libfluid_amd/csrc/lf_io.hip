@@ -60,6 +60,7 @@ int Pool::load_state(uint32_t id, const float *in, uint64_t steps) {
     (void)hipFree(d);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "load_state: %s", hipGetErrorString(e));
     s->identity = true;
+    s->fields = false;
     s->steps = steps;
     return LF_OK;
 }

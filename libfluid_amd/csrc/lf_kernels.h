@@ -42,15 +42,14 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
                     uint32_t *far_move);
 // The neighbour passes take `flat`: walk the masks as one flattened stream
 // of set bits (where the rows fit) rather than row by row; results are the
-// same either way.  Only scenes without own_normal are processed by
-// launch_normals.
-void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Q, const uint32_t *start,
-                    RowMasks M, float4 *N);
+// same either way.  launch_normals processes the scenes without lazy_normal,
+// or with `all` every scene (the on-demand normals of the last substep).
+void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool all, const float4 *Q,
+                    const uint32_t *start, RowMasks M, float4 *N);
 void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
-                   const float *Pr,
-                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
-                   const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
-                   float4 *X, float4 *Fdbg, uint32_t *far_move);
+                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
+                   const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout,
+                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *far_move);
 void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
                  const float4 *Vh, const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout);
 

@@ -478,9 +478,10 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
 // 6. surface normals (computeNormals, fluidphysics.cu:154-210):
 // n_i = h * sum m gradPoly6(r) (x_i - x_j) / (rho_j r); r cancels against
 // gradPoly6's factor r.  Neighbours with density <= FLT_MIN (Q.w < 0) or at
-// r^2 <= FLT_MIN (or passed with ok = false) contribute zero.  normal_term is shared by k_normals and by
-// k_forces, which computes the normals inside the force walk for scenes
-// without surface tension (own_normal); both give the same bits.
+// r^2 <= FLT_MIN (or passed with ok = false) contribute zero.  Only the
+// surface-tension force reads the normals: for a scene without it
+// (lazy_normal) the pass runs only when the normals are read, over the last
+// substep's data (all = 1), and gives the bits it would have given then.
 __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, bool ok, float &sx,
                                             float &sy, float &sz) {
     float t = fmaxf(c.h2 - rr, 0.0f);
@@ -494,11 +495,11 @@ __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q
 template <int BS>
 __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Q,
-                                                const uint32_t *__restrict__ start, RowMasks M, int flat,
+                                                const uint32_t *__restrict__ start, RowMasks M, int flat, int all,
                                                 float4 *__restrict__ N) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
-    if (!cp || cp->own_normal) return;
+    if (!cp || (cp->lazy_normal && !all)) return;
     const Consts &c = *cp;
     const uint32_t i = slot0 + threadIdx.x;
     if (i >= c.first + c.count) return;
@@ -525,14 +526,14 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
 // still integrated.  Fdbg (may be null) receives the four force components.
 // Writes X = (integrated position, density) for the XSPH pass and sets the
 // scene's far_move flag when a particle moved farther than the XSPH skin
-// masks allow.  For own_normal scenes the walk also computes the normals.
+// masks allow.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                uint32_t nslots, const float4 *__restrict__ Ps,
                                                const float4 *__restrict__ Q, const float *__restrict__ Pr,
                                                const float4 *__restrict__ Vs, const float4 *__restrict__ As,
                                                const float4 *__restrict__ Ls, const float4 *__restrict__ D,
-                                               float4 *__restrict__ N, const uint32_t *__restrict__ start,
+                                               const float4 *__restrict__ N, const uint32_t *__restrict__ start,
                                                RowMasks M, int flat, float4 *__restrict__ Pout,
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
                                                float4 *__restrict__ Lout, float4 *__restrict__ X,
@@ -545,18 +546,16 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     if (i >= c.first + c.count) return;
     const uint32_t *st = start + c.cell_base;
     const uint32_t shift = c.first - st[0];
-    const bool own_normal = c.own_normal != 0;
     const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
     const float4 di = D[i];
     float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
     if (i < st[c.ncells] + shift) {
-        const float4 ni = own_normal ? make_float4(0.f, 0.f, 0.f, 0.f) : N[i];
+        const float4 ni = c.has_tension ? N[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
         const float iri2 = iri * iri;
         float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
         float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
         float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
-        float nx = 0.f, ny = 0.f, nz = 0.f;  // own normal
         uint32_t b[9], e[9];
         rows9(c, st, shift, pi, b, e);
         // One walk per combination of the scene's optional terms, so that a
@@ -586,7 +585,6 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     const float4 q = cj.q;
                     const float pj = cj.p;
                     const float rr = dist2(pi, q);
-                    if (own_normal) normal_term(c, pi, q, rr, ok, nx, ny, nz);
                     if (!(ok && rr <= c.h2 && rr > c.eps)) return;
                     if (!(q.w >= 0.0f)) return;  // neighbour density <= FLT_MIN
                     const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
@@ -622,7 +620,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     }
                 });
         };
-        const bool tension = !own_normal && c.has_tension;
+        const bool tension = c.has_tension != 0;
         if (c.has_visc) {
             if (tension)
                 walk.template operator()<true, true>();
@@ -634,7 +632,6 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
             else
                 walk.template operator()<false, false>();
         }
-        if (own_normal) N[i] = make_float4(c.hmCg6 * nx, c.hmCg6 * ny, c.hmCg6 * nz, 0.0f);
         fP = make_float3(Px, Py, Pz);
         fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
         // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
@@ -651,8 +648,6 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 fA = make_float3((float)(s * nbv[0]), (float)(s * nbv[1]), (float)(s * nbv[2]));
             }
         }
-    } else if (own_normal) {
-        N[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (Fdbg) {
         Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
@@ -1048,17 +1043,17 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
     });
 }
 
-void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Q, const uint32_t *start,
-                    RowMasks M, float4 *N) {
+void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool all, const float4 *Q,
+                    const uint32_t *start, RowMasks M, float4 *N) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat ? 1 : 0,
-              N);
+              all ? 1 : 0, N);
     });
 }
 
 void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
-                   const float *Pr,
-                   const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D, float4 *N,
+                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
+                   const float4 *N,
                    const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
                    float4 *X, float4 *Fdbg, uint32_t *far_move) {
     with_bs(bs, [&]<int BS>() {

@@ -40,6 +40,11 @@ struct SceneRec {
     float dt = 0.0f;           // substep length
     bool identity = true;      // state arrays in particle-id order
     uint64_t steps = 0;
+    // The last substep's working data (sorted positions, densities, masks,
+    // cell table) are still in place: set by a substep that stepped this
+    // scene; cleared by a substep without it, compaction, new parameters or
+    // a loaded state.
+    bool fields = false;
 };
 
 class Pool {

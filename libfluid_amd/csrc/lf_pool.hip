@@ -105,7 +105,7 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.has_adhesion = p.adhesion != 0.0f;
     // Normals feed only the surface-tension term; without it the force walk
     // computes them itself (same bits) and the separate pass is not needed.
-    c.own_normal = (!c.has_tension && !(debug_flags & LF_DEBUG_NORMALS_PASS)) ? 1 : 0;
+    c.lazy_normal = (!c.has_tension && !(debug_flags & LF_DEBUG_NORMALS_PASS)) ? 1 : 0;
     c.m = m;
     c.dt = (double)dt;
     c.dtdt = (double)dt * (double)dt;
@@ -337,6 +337,7 @@ int Pool::compact() {
         scenes_[id].cell_base = cb;  // counts are zero between substeps
         cb += scenes_[id].ncells + 1;
     }
+    for (auto &s : scenes_) s.fields = false;
     tables_dirty_ = true;
     return LF_OK;
 }
@@ -406,6 +407,7 @@ int Pool::set_params(uint32_t id, const lf_params &p) {
         if (p.grid[a] != o.grid[a] || p.inv_cell[a] != o.inv_cell[a] || p.box[a] != o.box[a])
             return fail(LF_ERR_INVALID, "set_params: the grid and box are fixed when the scene is created");
     scenes_[id].params = p;
+    scenes_[id].fields = false;
     tables_dirty_ = true;
     drop_graph();  // the normals pass may come or go
     return LF_OK;
@@ -468,7 +470,7 @@ void Pool::timed(int kind, Fn &&launch) {
 bool Pool::any_separate_normals() const {
     for (uint32_t id = 0; id < scenes_.size(); ++id) {
         const SceneRec &s = scenes_[id];
-        if (s.live && s.active && !make_consts(s.params, s.dt, debug_).own_normal) return true;
+        if (s.live && s.active && !make_consts(s.params, s.dt, debug_).lazy_normal) return true;
     }
     return false;
 }
@@ -495,7 +497,7 @@ void Pool::enqueue_substep() {
           [&](const Launch &L) { launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, far_move_); });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
-            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, Q_, start_, M_, N_);
+            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, false, Q_, start_, M_, N_);
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
         launch_forces(L, pv, tune_.forces, tune_.flat_forces != 0, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
@@ -559,11 +561,13 @@ int Pool::step(uint32_t substeps) {
         }
     if (!any) return LF_OK;
     if (int rc = run_substeps(substeps)) return rc;
-    for (auto &s : scenes_)
+    for (auto &s : scenes_) {
+        s.fields = s.live && s.active;
         if (s.live && s.active) {
             s.steps += substeps;
             s.identity = false;
         }
+    }
     return LF_OK;
 }
 
@@ -700,7 +704,20 @@ int Pool::get_field(uint32_t id, int field, void *dst, size_t bytes) {
         case LF_FIELD_START: src = start_ + s->cell_base; need = 4 * ((size_t)s->ncells + 2); break;
         case LF_FIELD_SORTED_IDS: need = 4 * n; break;
         case LF_FIELD_DENSITY: src = D_ + f; need = 16 * n; break;
-        case LF_FIELD_NORMAL: src = N_ + f; need = 16 * n; break;
+        case LF_FIELD_NORMAL:
+            // Normals feed only the surface-tension force; scenes without it
+            // skip them while stepping.  They are computed here, for every
+            // scene, from the last substep's data by the same pass.
+            if (!s->fields)
+                return fail(LF_ERR_STATE, "normals: no substep data for this scene (none since it was added, "
+                                          "loaded, moved, re-parametrised or paused, or autotuning ran since)");
+            launch_normals(Launch{stream_, nullptr, nullptr}, view(), tune_.normals, tune_.flat_normals != 0, true,
+                           Q_, start_, M_, N_);
+            if (int rc = check_launch("normals")) return rc;
+            LF_HIP(hipStreamSynchronize(stream_));
+            src = N_ + f;
+            need = 16 * n;
+            break;
         case LF_FIELD_FORCES:
             if (!F_) return fail(LF_ERR_STATE, "force components are only kept after set_debug(LF_DEBUG_FORCES)");
             src = F_ + 4 * f;
@@ -953,6 +970,7 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     hipError_t e = hipStreamSynchronize(stream_);
     (void)hipFree(save);
     scenes_ = keep;
+    for (auto &s : scenes_) s.fields = false;  // the working data are the timing runs
     if (!rc && e != hipSuccess) rc = fail(LF_ERR_HIP, "autotune: %s", hipGetErrorString(e));
     if (rc) {
         tune_ = before;

@@ -108,8 +108,9 @@ class Scene:
     def set_debug(self, forces: bool = True, scan: bool = False, normals_pass: bool = False) -> None:
         """forces: keep the four force components; scan: test every stencil
         slot in the normals, forces and XSPH passes instead of walking the
-        density pass's neighbour masks; normals_pass: run the normals as their
-        own pass even without surface tension."""
+        density pass's neighbour masks; normals_pass: run the normals pass
+        every substep even without surface tension (otherwise they are
+        computed when read)."""
         flags = (L.DEBUG_FORCES if forces else 0) | (L.DEBUG_SCAN if scan else 0)
         flags |= L.DEBUG_NORMALS_PASS if normals_pass else 0
         check(lib.lf_set_debug(self._h, flags), "lf_set_debug")

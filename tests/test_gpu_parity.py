@@ -278,9 +278,11 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
             assert np.array_equal(outs[0][k], outs[m][k]), f"{case}: {k} differs between {name} and full scans"
 
 
-def test_normals_in_force_walk_match_separate_pass(gpu):
-    """Without surface tension the force walk computes the normals itself;
-    they and the trajectory are bitwise those of the separate normals pass."""
+def test_lazy_normals_match_per_substep_pass(gpu):
+    """Without surface tension the normals are skipped while stepping and
+    computed when read; the trajectory is bitwise that with the normals pass
+    run every substep, and the normals read are the same (also after the
+    state was brought back to id order)."""
     cfg = configs.dambreak_c1()
     outs = []
     for separate in (False, True):
@@ -290,10 +292,26 @@ def test_normals_in_force_walk_match_separate_pass(gpu):
             sc.set_debug(forces=False, normals_pass=separate)
             sc.run(cfg.dt, 10)
             sc.sync()
+            n_before = sc.field(L.FIELD_NORMAL)
             d = sc.download(L.LF_ORDER_ID)
             d["normal"] = sc.field(L.FIELD_NORMAL)
+            assert np.array_equal(d["normal"], n_before)
+            assert np.any(n_before[:, :3] != 0.0)
             outs.append(d)
         finally:
             sc.close()
     for k in ("pos", "vel", "acc", "colour", "normal"):
-        assert np.array_equal(outs[0][k], outs[1][k]), f"{k} differs between fused and separate normals"
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{k} differs between lazy and per-substep normals"
+
+
+def test_normals_need_substep_data(gpu):
+    """Reading the normals before any substep (or after the scene was
+    re-parametrised) is reported, not answered with stale data."""
+    cfg = configs.scaled_c1(8, 8, 8)
+    with gpu.Scene(cfg.params, cfg.n) as sc:
+        sc.upload(cfg.pos, cfg.vel)
+        with pytest.raises(L.LfError):
+            sc.field(L.FIELD_NORMAL)
+        sc.run(cfg.dt, 2)
+        sc.sync()
+        assert sc.field(L.FIELD_NORMAL).shape == (cfg.n, 4)
