@@ -293,37 +293,48 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 // mask's low and high words, the second starting 32 slots later), compacts
 // the non-empty ones, (mask, first slot), into its own column of the
 // workgroup's LDS table tab[kFlatEntries][BS] and walks them in row order,
-// two set bits per step (the two may come from adjacent pseudo-rows).  The
-// next entry is fetched one entry ahead, so a row change does not wait on
-// LDS.  Visits the same slots in the same order as walk_masks.  Returns
-// false, having visited nothing, when a lane of the wave has a row of more
-// than 64 slots or more than kFlatEntries non-empty pseudo-rows.
+// W set bits per step (consecutive bits may come from adjacent pseudo-rows):
+// the W candidates' loads are all issued before the first is used, so W
+// gathers per lane are in flight.  The next entry is fetched one entry
+// ahead, so a row change does not wait on LDS.  Visits the same slots in the
+// same order as walk_masks.  Returns false, having visited nothing, when a
+// lane of the wave has a row of more than 64 slots or more than kFlatEntries
+// non-empty pseudo-rows.
 constexpr int kFlatEntries = 16;
-template <int BS, class Ld, class Fn>
+constexpr int kForceWalkWidth = 4;  // set bits per step of the flattened force walk
+template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
     const uint32_t t = threadIdx.x;
+    // Each row's mask with one unconditional 8-byte load (from the 64-bit
+    // array for a row of 33..64 slots, else from the 32-bit array, whose
+    // allocation is padded so that the word after the last entry is
+    // readable; that second word is then dropped), so the nine loads are in
+    // flight together.
+    uint64_t mw[9];
+    bool fits = true;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const uint32_t len = e[r] - b[r];
+        const size_t at = (size_t)r * nslots + i;
+        const char *src = len > 32u ? reinterpret_cast<const char *>(mask.w64 + at)
+                                    : reinterpret_cast<const char *>(mask.w32 + at);
+        uint64_t v;
+        __builtin_memcpy(&v, src, sizeof(v));
+        mw[r] = len > 32u ? v : (v & 0xFFFFFFFFull);
+        fits = fits && len <= 64u;
+    }
     // An empty pseudo-row is written at the running count and overwritten by
     // the next; entries at or past cnt are never used.
     uint32_t cnt = 0u;
-    bool fits = true;
     auto put = [&](uint32_t m, uint32_t base) {
         tab[(cnt < (uint32_t)kFlatEntries ? cnt : 0u) * BS + t] = make_uint2(m, base);
         cnt += m != 0u ? 1u : 0u;
     };
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-        const uint32_t len = e[r] - b[r];
-        const size_t at = (size_t)r * nslots + i;
-        if (len <= 32u) {
-            put(mask.w32[at], b[r]);
-        } else if (len <= 64u) {
-            const uint64_t m = mask.w64[at];
-            put((uint32_t)m, b[r]);
-            put((uint32_t)(m >> 32), b[r] + 32u);
-        } else {
-            fits = false;
-        }
+        put((uint32_t)mw[r], b[r]);
+        put((uint32_t)(mw[r] >> 32), b[r] + 32u);
     }
     if (__ballot(!fits || cnt > (uint32_t)kFlatEntries)) return false;
     const uint2 e0 = tab[t];
@@ -337,19 +348,26 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
         nx = tab[(nxt < (uint32_t)kFlatEntries ? nxt : 0u) * BS + t];
     };
     while (m) {
-        const uint32_t j1 = base + (uint32_t)__builtin_ctz(m);
+        uint32_t j[W];
+        bool ok[W];
+        j[0] = base + (uint32_t)__builtin_ctz(m);
+        ok[0] = true;
         m &= m - 1u;
         if (!m) advance();
-        const bool two = m != 0u;
-        const uint32_t j2 = two ? base + (uint32_t)__builtin_ctz(m) : j1;
-        if (two) {
-            m &= m - 1u;
-            if (!m) advance();
+#pragma unroll
+        for (int k = 1; k < W; ++k) {  // once a bit is missing, so are the rest
+            ok[k] = m != 0u;
+            j[k] = ok[k] ? base + (uint32_t)__builtin_ctz(m) : j[0];
+            if (ok[k]) {
+                m &= m - 1u;
+                if (!m) advance();
+            }
         }
-        const auto a1 = ld(j1);
-        const auto a2 = ld(j2);
-        fn(a1, j1, true);
-        fn(a2, j2, two);
+        decltype(ld(0u)) a[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) a[k] = ld(j[k]);
+#pragma unroll
+        for (int k = 0; k < W; ++k) fn(a[k], j[k], ok[k]);
     }
     return true;
 }
@@ -357,11 +375,11 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
 // Walk over the neighbour slots of lane i: with flat set, the flattened walk
 // when masks are in use and every lane's rows fit it, else walk_masks.  The
 // choice is wave-uniform and does not change the result.
-template <int BS, class Ld, class Fn>
+template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, bool use, bool flat, uint32_t nslots, uint32_t i,
                                           Ld &&ld, Fn &&fn) {
-    if (use && flat && walk_flat<BS>(tab, b, e, mask, nslots, i, ld, fn)) return;
+    if (use && flat && walk_flat<BS, W>(tab, b, e, mask, nslots, i, ld, fn)) return;
     walk_masks(b, e, mask, use, nslots, i, ld, fn);
 }
 
@@ -514,7 +532,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
     rows9(c, st, shift, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     __shared__ uint2 tab[kFlatEntries * BS];
-    walk_rows<BS>(
+    walk_rows<BS, 2>(
         tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
         [&](const float4 &q, uint32_t, bool ok) { normal_term(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
@@ -568,7 +586,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 float p;
                 float4 v, d, n;
             };
-            walk_rows<BS>(
+            walk_rows<BS, kForceWalkWidth>(
                 tab, b, e, M, c.use_masks, flat != 0, nslots, i,
                 [&](uint32_t j) {
                     Cand cd;
@@ -751,7 +769,7 @@ __global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, cons
         float4 x, v;
     };
     __shared__ uint2 tab[kFlatEntries * BS];
-    walk_rows<BS>(
+    walk_rows<BS, 2>(
         tab, b, e, MS, masks_ok, flat != 0, nslots, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
         [&](const Cand &cj, uint32_t, bool ok) {
             const float4 q = cj.x;

@@ -177,9 +177,9 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         if (!rc) rc = dalloc(b, n);
     if (!rc) rc = dalloc(&Pr_, n);
     if (!rc) rc = dalloc(&soa_, 3 * n);
-    if (!rc) rc = dalloc(&M_.w32, 9 * n);
+    if (!rc) rc = dalloc(&M_.w32, 9 * n + 2);  // +2: an 8-byte read at the last entry stays inside
     if (!rc) rc = dalloc(&M_.w64, 9 * n);
-    if (!rc) rc = dalloc(&MS_.w32, 9 * n);
+    if (!rc) rc = dalloc(&MS_.w32, 9 * n + 2);
     if (!rc) rc = dalloc(&MS_.w64, 9 * n);
     uint32_t **u32[] = {&keys_, &ranks_, &slot_src_, &slot_id_};
     for (uint32_t **b : u32)
