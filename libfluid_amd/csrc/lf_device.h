@@ -63,6 +63,15 @@ struct Consts {
     // integrator
     double g[3], m, dt, dtdt, vlim, vlim2, alim, alim2;
     double vdamp1, psize, h_d, bd_rho0_d;
+    // Near-isolated particles (density below rho0/64: few neighbours, all
+    // far out) turn the FP32 cancellation in h^2 - r^2 as r -> h into large
+    // relative errors of huge values (1/rho^2).  Pairs touching one take the
+    // pair geometry in FP64: hh_d = h^2, mC6_d = m * C6; fragile_S bounds the
+    // FP32 sum of (h^2 - r^2)^3 below which the density is recomputed;
+    // fragile_ir is the 1/rho above which a particle is near-isolated,
+    // fragile_sr the rho_i + rho_j below which an XSPH pair is.
+    double hh_d, mC6_d;
+    float fragile_S, fragile_ir, fragile_sr;
     // XSPH
     float xsph_k;  // xsph * 2 m * C6
 };
@@ -74,6 +83,25 @@ __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.
 __device__ __forceinline__ float dist2(float4 a, float4 b) {
     float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Per-scene flag word of the substep, one byte per flag, so that the kernels
+// setting and clearing different flags store to different bytes (plain
+// stores, no read-modify-write) and a reader loads the whole word once:
+// far move: a particle moved farther than the XSPH skin masks allow (set by
+// k_forces, cleared by k_density); near-isolated: a particle's density is
+// below rho0/64 (set by k_density, cleared by k_hash).
+constexpr uint32_t kFarMoveByte = 0u, kFragileByte = 1u;
+__device__ __forceinline__ void set_flag(uint32_t *flags, uint32_t scene, uint32_t byte, uint8_t v) {
+    reinterpret_cast<uint8_t *>(flags + scene)[byte] = v;
+}
+__device__ __forceinline__ bool flag_set(uint32_t word, uint32_t byte) { return ((word >> (8u * byte)) & 0xFFu) != 0u; }
+
+// |b - a|^2 in FP64 from FP32 positions: the differences and squares are
+// exact, so the result is accurate to FP64 rounding.
+__device__ __forceinline__ double dist2_d(float4 a, float4 b) {
+    const double dx = (double)b.x - (double)a.x, dy = (double)b.y - (double)a.y, dz = (double)b.z - (double)a.z;
+    return dx * dx + dy * dy + dz * dz;
 }
 
 // Cell coordinate along one axis: floor(x * res) (fluidutil.cu:15-18).

@@ -47,12 +47,14 @@ namespace lf {
 // out consecutive ranks.  Counts are zeroed by the previous substep's scan.
 __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                  const float4 *__restrict__ P, uint32_t *__restrict__ keys,
-                                                 uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts) {
+                                                 uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts,
+                                                 uint32_t *__restrict__ flags) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
     const uint32_t i = slot0 + threadIdx.x;
+    if (i == c.first) set_flag(flags, c.scene, kFragileByte, 0u);  // set by k_density
     const int lane = threadIdx.x & 63;
     const bool active = i < c.first + c.count;
     const uint32_t b = active ? c.cell_base + bucket_of(c, P[i]) : 0xFFFFFFFFu;
@@ -386,20 +388,22 @@ __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_
 // 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
 // neighbour masks of the substep and the packed neighbour data of the later
 // passes: Q = (x, y, z, 1/density, or -1 when density <= FLT_MIN), Pr =
-// pressure.  Clears the scene's far_move flag.
+// pressure.  Clears the scene's far-move flag; sets its near-isolated flag
+// when a particle's density is below rho0/64, so that the later passes take
+// their FP64 pair-geometry walks for the scene.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Ps,
                                                 const float *__restrict__ soa, const uint32_t *__restrict__ start,
                                                 float4 *__restrict__ D,
                                                 float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
-                                                RowMasks MS, uint32_t *__restrict__ far_move) {
+                                                RowMasks MS, uint32_t *__restrict__ flags) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
     const uint32_t i = slot0 + threadIdx.x;
-    if (i == c.first) far_move[c.scene] = 0u;  // set by k_forces when the skin is exceeded
+    if (i == c.first) set_flag(flags, c.scene, kFarMoveByte, 0u);  // set by k_forces when the skin is exceeded
     if (i >= c.first + c.count) return;
     const uint32_t *st = start + c.cell_base;
     const uint32_t shift = c.first - st[0];
@@ -473,7 +477,35 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             MS.w64[at] = (uint64_t)(__brev(s1) >> sh1) << 32 | s0;
         }
     }
-    S *= c.mC6;  // sum of m * poly6
+    if (S < c.fragile_S) {
+        // Near-isolated: the same pairs again, in the same order, with the
+        // pair geometry in FP64 (the FP32 h^2 - r^2 of a pair near r = h
+        // carries a large relative error, and here few pairs make the sum).
+        double Sd = 0.0;
+        for (int r = 0; r < 9; ++r) {
+            const uint32_t len = e[r] - b[r];
+            const size_t at = (size_t)r * nslots + i;
+            if (len <= 64u) {
+                uint64_t m = len <= 32u ? (uint64_t)M.w32[at] : M.w64[at];
+                while (m) {
+                    const uint32_t j = b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    m &= m - 1ull;
+                    const double t = fmax(c.hh_d - dist2_d(pi, at32(Ps, j)), 0.0);
+                    Sd += t * t * t;
+                }
+            } else {
+                for (uint32_t j = b[r]; j < e[r]; ++j) {
+                    const float4 q = at32(Ps, j);
+                    if (j == i || !(dist2(pi, q) <= c.h2)) continue;
+                    const double t = fmax(c.hh_d - dist2_d(pi, q), 0.0);
+                    Sd += t * t * t;
+                }
+            }
+        }
+        S = (float)(Sd * c.mC6_d);
+    } else {
+        S *= c.mC6;  // sum of m * poly6
+    }
     // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
     // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.
     const double dist = fabs(box_sdf(c, pi.x, pi.y, pi.z));
@@ -488,6 +520,10 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (powf(ratio, c.gamma) - 1.0f);
     const float cs_i = c.K * powf(ratio, c.gamma_m1);
     const float ir = 1.0f / rho;
+    // Every pair the later passes treat as near-isolated (1/rho_j or 1/rho_i
+    // above fragile_ir; rho_i + rho_j below fragile_sr, which needs one
+    // density below fragile_sr / 2) has a particle that sets the flag here.
+    if (ir > c.fragile_ir || rho < 0.5f * c.fragile_sr) set_flag(flags, c.scene, kFragileByte, 1u);
     D[i] = make_float4(rho, press, cs_i, ir);
     Q[i] = make_float4(pi.x, pi.y, pi.z, rho > c.eps ? ir : -1.0f);
     Pr[i] = press;
@@ -500,10 +536,17 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
 // surface-tension force reads the normals: for a scene without it
 // (lazy_normal) the pass runs only when the normals are read, over the last
 // substep's data (all = 1), and gives the bits it would have given then.
+// F: the scene has near-isolated particles; a pair with one takes h^2 - r^2
+// in FP64.
+template <bool F>
 __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q, float rr, bool ok, float &sx,
                                             float &sy, float &sz) {
     float t = fmaxf(c.h2 - rr, 0.0f);
     t = ok && rr > c.eps ? t : 0.0f;
+    if constexpr (F) {
+        if (t > 0.0f && q.w > c.fragile_ir)  // near-isolated neighbour: FP64 pair geometry
+            t = (float)fmax(c.hh_d - dist2_d(pi, q), 0.0);
+    }
     const float w = t * t * fmaxf(q.w, 0.0f);
     sx += w * (pi.x - q.x);
     sy += w * (pi.y - q.y);
@@ -514,7 +557,7 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Q,
                                                 const uint32_t *__restrict__ start, RowMasks M, int flat, int all,
-                                                float4 *__restrict__ N) {
+                                                const uint32_t *__restrict__ flags, float4 *__restrict__ N) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp || (cp->lazy_normal && !all)) return;
@@ -532,9 +575,15 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
     rows9(c, st, shift, pi, b, e);
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     __shared__ uint2 tab[kFlatEntries * BS];
-    walk_rows<BS, 2>(
-        tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
-        [&](const float4 &q, uint32_t, bool ok) { normal_term(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
+    auto walk = [&]<bool F>() {
+        walk_rows<BS, 2>(
+            tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
+            [&](const float4 &q, uint32_t, bool ok) { normal_term<F>(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
+    };
+    if (__builtin_expect(flag_set(flags[c.scene], kFragileByte), 0))
+        walk.template operator()<true>();
+    else
+        walk.template operator()<false>();
     N[i] = make_float4(c.hmCg6 * sx, c.hmCg6 * sy, c.hmCg6 * sz, 0.0f);
 }
 
@@ -543,7 +592,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
 // Writes X = (integrated position, density) for the XSPH pass and sets the
-// scene's far_move flag when a particle moved farther than the XSPH skin
+// scene's far-move flag when a particle moved farther than the XSPH skin
 // masks allow.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
@@ -555,7 +604,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                                                RowMasks M, int flat, float4 *__restrict__ Pout,
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
                                                float4 *__restrict__ Lout, float4 *__restrict__ X,
-                                               float4 *__restrict__ Fdbg, uint32_t *__restrict__ far_move) {
+                                               float4 *__restrict__ Fdbg, uint32_t *__restrict__ flags) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
@@ -578,9 +627,13 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
         rows9(c, st, shift, pi, b, e);
         // One walk per combination of the scene's optional terms, so that a
         // candidate's loads (V: velocity and density/sound speed; T: normal)
-        // are all issued with its position before any arithmetic.
+        // are all issued with its position before any arithmetic.  F: the
+        // scene has near-isolated particles, and a pair with one takes its
+        // h - r in FP64 (a separate walk, so that the other scenes' walks
+        // carry none of its registers' cost).
         __shared__ uint2 tab[kFlatEntries * BS];
-        auto walk = [&]<bool V, bool T>() {
+        const bool fragile_i = iri > c.fragile_ir;
+        auto walk = [&]<bool V, bool T, bool F>() {
             struct Cand {
                 float4 q;
                 float p;
@@ -608,9 +661,19 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     const float rx = q.x - pi.x, ry = q.y - pi.y, rz = q.z - pi.z;  // x_j - x_i
                     const float inv_r = frsq(rr);
                     const float r = rr * inv_r;
-                    const float hr = fmaxf(c.h - r, 0.0f);
-                    const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
                     const float irj = q.w;
+                    float hr = fmaxf(c.h - r, 0.0f);
+                    if constexpr (F) {
+                        if (fragile_i || irj > c.fragile_ir) {
+                            // Near-isolated pair: h - r in FP64, r from the
+                            // FP64 r^2 by one Newton step from the FP32 r
+                            // (relative error squared: ~1e-14).
+                            const double d2 = dist2_d(pi, q), r0 = (double)r;
+                            const double rd = __builtin_fma(__builtin_fma(-r0, r0, d2), (double)(0.5f * inv_r), r0);
+                            hr = (float)fmax(c.h_d - rd, 0.0);
+                        }
+                    }
+                    const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
                     const float mid = pj * iri2 + p_i * (irj * irj);
                     const float cpr = cm * mid;
                     Px += cpr * rx;
@@ -639,17 +702,23 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 });
         };
         const bool tension = c.has_tension != 0;
-        if (c.has_visc) {
-            if (tension)
-                walk.template operator()<true, true>();
-            else
-                walk.template operator()<true, false>();
-        } else {
-            if (tension)
-                walk.template operator()<false, true>();
-            else
-                walk.template operator()<false, false>();
-        }
+        auto terms = [&]<bool F>() {
+            if (c.has_visc) {
+                if (tension)
+                    walk.template operator()<true, true, F>();
+                else
+                    walk.template operator()<true, false, F>();
+            } else {
+                if (tension)
+                    walk.template operator()<false, true, F>();
+                else
+                    walk.template operator()<false, false, F>();
+            }
+        };
+        if (__builtin_expect(flag_set(flags[c.scene], kFragileByte), 0))
+            terms.template operator()<true>();
+        else
+            terms.template operator()<false>();
         fP = make_float3(Px, Py, Pz);
         fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
         // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
@@ -726,7 +795,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     // are exact in FP64).  Every writer stores the same value; NaN counts
     // as too far.
     const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
-    if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) far_move[c.scene] = 1u;
+    if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) set_flag(flags, c.scene, kFarMoveByte, 1u);
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
@@ -736,13 +805,14 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
 // a pair within h after the move was within h + 2 max|dx| before it, so the
 // masks are complete while no particle moved farther than move_limit
 // (h + 2 move_limit <= h_s with FP32 margins); k_forces flags a farther move
-// in the scene's far_move flag, and then every row is scanned.
+// in the scene's far-move flag, and then every row is scanned.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                              uint32_t nslots, const float4 *__restrict__ Ps,
                                              const float4 *__restrict__ X, const float4 *__restrict__ Vh,
-                                             const uint32_t *__restrict__ start, RowMasks MS, int flat,
-                                             const uint32_t *__restrict__ far_move, float4 *__restrict__ Vout) {
+                                             const float4 *__restrict__ Lw, const uint32_t *__restrict__ start,
+                                             RowMasks MS, int flat, const uint32_t *__restrict__ flags,
+                                             float4 *__restrict__ Vout) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
@@ -756,7 +826,8 @@ __global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, cons
         Vout[i] = vi;
         return;
     }
-    const bool masks_ok = c.use_masks && far_move[c.scene] == 0u;
+    const uint32_t fl = flags[c.scene];
+    const bool masks_ok = c.use_masks && !flag_set(fl, kFarMoveByte);
     const float4 xi = X[i];
     const float rho_i = xi.w;
     uint32_t b[9], e[9];
@@ -765,24 +836,45 @@ __global__ __launch_bounds__(BS) void k_xsph(const Consts *__restrict__ cs, cons
     const float h2 = c.h2, eps = c.eps;
     // Both loads of a candidate are issued together; a candidate outside h
     // (or with rho_i + rho_j <= FLT_MIN) adds exact zeros, branch-free.
+    // F: the scene has near-isolated particles (a separate walk, so that the
+    // other scenes' walks carry none of its registers' cost).
     struct Cand {
         float4 x, v;
     };
     __shared__ uint2 tab[kFlatEntries * BS];
-    walk_rows<BS, 2>(
-        tab, b, e, MS, masks_ok, flat != 0, nslots, i, [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
-        [&](const Cand &cj, uint32_t, bool ok) {
-            const float4 q = cj.x;
-            const float4 vj = cj.v;
-            const float rr = dist2(xi, q);
-            const float sr = rho_i + q.w;
-            const uint64_t in = lanes(rr <= h2) & lanes(sr > eps) & lanes(ok);
-            const float t = keep(h2 - rr, in);
-            const float w = keep(t * t * t * frcp(sr), in);
-            sx += w * (vj.x - vi.x);
-            sy += w * (vj.y - vi.y);
-            sz += w * (vj.z - vi.z);
-        });
+    auto walk = [&]<bool F>() {
+        walk_rows<BS, 2>(
+            tab, b, e, MS, masks_ok, flat != 0, nslots, i,
+            [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
+            [&](const Cand &cj, uint32_t j, bool ok) {
+                const float4 q = cj.x;
+                const float4 vj = cj.v;
+                const float rr = dist2(xi, q);
+                const float sr = rho_i + q.w;
+                const uint64_t in = lanes(rr <= h2) & lanes(sr > eps) & lanes(ok);
+                float t = keep(h2 - rr, in);
+                if constexpr (F) {
+                    if (sr < c.fragile_sr) {
+                        // Near-isolated pair: h^2 - r^2 in FP64 on the
+                        // double-float integrated positions (high word in X,
+                        // low word in Lw).
+                        const float4 li = Lw[i], lj = at32(Lw, j);
+                        const double dx = ((double)q.x + (double)lj.x) - ((double)xi.x + (double)li.x);
+                        const double dy = ((double)q.y + (double)lj.y) - ((double)xi.y + (double)li.y);
+                        const double dz = ((double)q.z + (double)lj.z) - ((double)xi.z + (double)li.z);
+                        t = keep((float)fmax(c.hh_d - (dx * dx + dy * dy + dz * dz), 0.0), in);
+                    }
+                }
+                const float w = keep(t * t * t * frcp(sr), in);
+                sx += w * (vj.x - vi.x);
+                sy += w * (vj.y - vi.y);
+                sz += w * (vj.z - vi.z);
+            });
+    };
+    if (__builtin_expect(flag_set(fl, kFragileByte), 0))
+        walk.template operator()<true>();
+    else
+        walk.template operator()<false>();
     Vout[i] = make_float4(vi.x + c.xsph_k * sx, vi.y + c.xsph_k * sy, vi.z + c.xsph_k * sz, 0.0f);
 }
 
@@ -1027,8 +1119,8 @@ static inline void with_bs(int bs, F &&body) {
 bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
-                 uint32_t *counts) {
-    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts);
+                 uint32_t *counts, uint32_t *flags) {
+    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
@@ -1054,18 +1146,18 @@ void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
-                    uint32_t *far_move) {
+                    uint32_t *flags) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
-              MS, far_move);
+              MS, flags);
     });
 }
 
 void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool all, const float4 *Q,
-                    const uint32_t *start, RowMasks M, float4 *N) {
+                    const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat ? 1 : 0,
-              all ? 1 : 0, N);
+              all ? 1 : 0, flags, N);
     });
 }
 
@@ -1073,18 +1165,19 @@ void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const
                    const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
                    const float4 *N,
                    const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
-                   float4 *X, float4 *Fdbg, uint32_t *far_move) {
+                   float4 *X, float4 *Fdbg, uint32_t *flags) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
-              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, far_move);
+              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, flags);
     });
 }
 
 void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
-                 const float4 *Vh, const uint32_t *start, RowMasks MS, const uint32_t *far_move, float4 *Vout) {
+                 const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
+                 float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, start, MS,
-              flat ? 1 : 0, far_move, Vout);
+        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
+              flat ? 1 : 0, flags, Vout);
     });
 }
 

@@ -32,7 +32,7 @@ constexpr double kPi = 3.141592654;  // CUDART_PI_F as defined by the reference 
 // v dt + a dt^2 / 2 with |a| <= acc_limit and |v| at most about vel_limit
 // (velocities are clamped to it before XSPH blends them), so the skin is
 // sized for both particles of a pair moving that far, with 10% to spare,
-// between 0.02 and 0.1.  A larger move sets the scene's far_move flag and
+// between 0.02 and 0.1.  A larger move sets the scene's far-move flag and
 // the XSPH pass scans (the result is the same; only the speed differs).
 double skin_of(const lf_params &p, double dt) {
     const double move = (double)p.vel_limit * dt + 0.5 * (double)p.acc_limit * dt * dt;
@@ -88,6 +88,12 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.K = p.gas_constant;
     c.bd_rho0_d = (double)p.rest_density * (double)0.0039062f;  // fluidkernel.cu:65
     c.h_d = h;
+    c.hh_d = h * h;
+    c.mC6_d = m * C6;
+    const double fragile_rho = (double)p.rest_density / 64.0;
+    c.fragile_S = m * C6 > 0.0 ? (float)(fragile_rho / (m * C6)) : 0.0f;
+    c.fragile_ir = (float)(1.0 / fragile_rho);
+    c.fragile_sr = (float)(2.0 * fragile_rho);
     c.hmCg6 = (float)(h * m * Cg6);
     c.mCgs = (float)(m * Cgs);
     c.nu = p.viscosity;
@@ -126,7 +132,7 @@ void Pool::release() {
     stop_exporter();
     drop_graph();
     void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, soa_, M_.w32, M_.w64, MS_.w32,
-                    MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, far_move_, d_cs_,
+                    MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, flags_, d_cs_,
                     d_chunk_};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -188,7 +194,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (!rc) rc = dalloc(&start_, (size_t)nbuckets_ + 1);
     if (!rc) rc = dalloc(&bsum_, (size_t)ntiles_ + 1);
     if (!rc) rc = dalloc(&err_, 1);
-    if (!rc) rc = dalloc(&far_move_, scene_cap);
+    if (!rc) rc = dalloc(&flags_, scene_cap);
     if (!rc) rc = dalloc(&d_cs_, scene_cap);
     if (!rc) rc = dalloc(&d_chunk_, h_chunk_.size());
     for (int k = 0; k < kTableRing && !rc; ++k) {
@@ -203,7 +209,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         e = hipMemsetAsync(counts_, 0, sizeof(uint32_t) * nbuckets_, stream_);
         if (e == hipSuccess) e = hipMemsetAsync(start_, 0, sizeof(uint32_t) * (nbuckets_ + 1), stream_);
         if (e == hipSuccess) e = hipMemsetAsync(err_, 0, sizeof(uint32_t), stream_);
-        if (e == hipSuccess) e = hipMemsetAsync(far_move_, 0, sizeof(uint32_t) * scene_cap, stream_);
+        if (e == hipSuccess) e = hipMemsetAsync(flags_, 0, sizeof(uint32_t) * scene_cap, stream_);
         if (e != hipSuccess) rc = fail(LF_ERR_HIP, "pool: %s", hipGetErrorString(e));
     }
     if (!rc) rc = push_tables();
@@ -479,7 +485,7 @@ bool Pool::any_separate_normals() const {
 // A0, L0) into cell order (P1, V1, A1, L1).
 void Pool::enqueue_sort() {
     const PoolView pv = view();
-    timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_); });
+    timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_); });
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, nbuckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
@@ -493,18 +499,19 @@ void Pool::enqueue_sort() {
 void Pool::enqueue_substep() {
     const PoolView pv = view();
     enqueue_sort();
-    timed(LF_K_DENSITY,
-          [&](const Launch &L) { launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, far_move_); });
+    timed(LF_K_DENSITY, [&](const Launch &L) {
+        launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, flags_);
+    });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
-            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, false, Q_, start_, M_, N_);
+            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, false, Q_, start_, M_, flags_, N_);
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
         launch_forces(L, pv, tune_.forces, tune_.flat_forces != 0, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
-                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, far_move_);
+                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
-        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph != 0, P1_, X_, V2_, start_, MS_, far_move_, V0_);
+        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph != 0, P1_, X_, V2_, L0_, start_, MS_, flags_, V0_);
     });
 }
 
@@ -712,7 +719,7 @@ int Pool::get_field(uint32_t id, int field, void *dst, size_t bytes) {
                 return fail(LF_ERR_STATE, "normals: no substep data for this scene (none since it was added, "
                                           "loaded, moved, re-parametrised or paused, or autotuning ran since)");
             launch_normals(Launch{stream_, nullptr, nullptr}, view(), tune_.normals, tune_.flat_normals != 0, true,
-                           Q_, start_, M_, N_);
+                           Q_, start_, M_, flags_, N_);
             if (int rc = check_launch("normals")) return rc;
             LF_HIP(hipStreamSynchronize(stream_));
             src = N_ + f;

