@@ -41,12 +41,12 @@ struct Consts {
     double half_box[3];
     // smoothing length
     float h, h2, inv_h, eps;
-    // neighbour masks: squared skin radius (>= h2) of the XSPH masks; the
-    // largest particle move (FP32 positions, FP64 distance) for which the
-    // skin masks still hold every XSPH neighbour; whether the normals, forces
+    // neighbour masks: skin_q = 2.2 * 4096 / h turns a particle's move into
+    // the skin fraction (in 1/4096) that covers both particles of a pair
+    // moving that far, with 10% to spare (the substep's skin radius follows
+    // from the scene's largest move: skin_h2s); whether the normals, forces
     // and XSPH passes walk masks at all (off: every row is scanned)
-    float h2s;
-    double move_limit;
+    float skin_q;
     int use_masks;
     // density + EOS
     float mC6;  // m * 315/(64 pi h^9)
@@ -85,17 +85,73 @@ __device__ __forceinline__ float dist2(float4 a, float4 b) {
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
-// Per-scene flag word of the substep, one byte per flag, so that the kernels
-// setting and clearing different flags store to different bytes (plain
-// stores, no read-modify-write) and a reader loads the whole word once:
-// far move: a particle moved farther than the XSPH skin masks allow (set by
-// k_forces, cleared by k_density); near-isolated: a particle's density is
-// below rho0/64 (set by k_density, cleared by k_hash).
+// Per-scene words of the substep, kSceneWords per scene.
+// Word 0 holds one byte per flag, so that the kernels setting and clearing
+// different flags store to different bytes (plain stores, no
+// read-modify-write) and a reader loads the whole word once: far move: a
+// particle moved farther than the XSPH skin masks allow (set by k_forces,
+// cleared by k_density); near-isolated: a particle's density is below
+// rho0/64 (set by k_density, cleared by k_hash).
+constexpr uint32_t kSceneWords = 1u;
 constexpr uint32_t kFarMoveByte = 0u, kFragileByte = 1u;
 __device__ __forceinline__ void set_flag(uint32_t *flags, uint32_t scene, uint32_t byte, uint8_t v) {
-    reinterpret_cast<uint8_t *>(flags + scene)[byte] = v;
+    reinterpret_cast<uint8_t *>(flags + kSceneWords * scene)[byte] = v;
+}
+__device__ __forceinline__ uint32_t flag_word(const uint32_t *flags, uint32_t scene) {
+    return flags[kSceneWords * scene];
 }
 __device__ __forceinline__ bool flag_set(uint32_t word, uint32_t byte) { return ((word >> (8u * byte)) & 0xFFu) != 0u; }
+
+// XSPH skin.  The substep moves a particle by v dt + a dt^2 / 2 (its
+// velocity and acceleration as they enter the substep; the wall projection
+// that follows does not lengthen a move that starts inside the walls), so a
+// pair within h after the move was within h + 2 max|move| before it.  The
+// gather pass reduces each scene's largest move, as the skin fraction
+// q / 4096 it needs (rounded up, plus one), into kMoveSlots words of the
+// scene's move table, one cache line apart (device-scope atomics on one
+// address serialise; each workgroup updates the slot of its index); the
+// density and force passes read the slots back (scene_move_q).  The skin
+// radius is
+// (1 + s) h with s = q / 4096 between kSkinMin and kSkinMax.  A particle
+// that still moves farther than skin_move_limit (a wall projection of a
+// particle that started outside, or a move above the kSkinMax skin) sets
+// the scene's far-move flag and the XSPH pass scans: the result is the same
+// either way, only the speed differs.
+constexpr uint32_t kSkinQMax = 512u;  // s = 1/8: above kSkinMax
+constexpr uint32_t kMoveSlots = 16u, kMoveStride = 32u;  // words per scene: kMoveSlots * kMoveStride
+__device__ __forceinline__ uint32_t *move_slot(uint32_t *moves, uint32_t scene, uint32_t k) {
+    return moves + ((size_t)scene * kMoveSlots + k) * kMoveStride;
+}
+constexpr double kSkinMin = 0.01, kSkinMax = 0.1;
+__device__ __forceinline__ uint32_t skin_quantum(const Consts &c, float4 v, float4 a) {
+    const float dt = (float)c.dt, hdt2 = 0.5f * (float)c.dtdt;
+    const float mx = __builtin_fmaf(a.x, hdt2, v.x * dt), my = __builtin_fmaf(a.y, hdt2, v.y * dt),
+                mz = __builtin_fmaf(a.z, hdt2, v.z * dt);
+    const float q = ceilf(sqrtf(__builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx))) * c.skin_q) + 1.0f;
+    return (uint32_t)fminf(q, (float)kSkinQMax);  // NaN -> kSkinQMax
+}
+// Skin fraction s (exact in FP32) for the scene's quantised largest move.
+__device__ __forceinline__ float skin_frac(uint32_t q) {
+    return fminf(fmaxf((float)q * (1.0f / 4096.0f), (float)kSkinMin), (float)kSkinMax);
+}
+// Squared skin radius of the masks, (1 + s)^2 h2 in FP32: sqrt(h2s) >=
+// (1 + s) h (1 - 2e-7) with the roundings of 1 + s, the products and
+// h2 = h * h.
+__device__ __forceinline__ float skin_h2s(const Consts &c, float s) {
+    const float k = 1.0f + s;
+    return k * k * c.h2;
+}
+// The largest move (FP32 positions, FP64 distance) for which masks built
+// with skin_h2s(s) still hold every XSPH neighbour: a pair within h after
+// the move (FP32 predicate: true distance below h (1 + 1e-6)) was within
+// h (1 + 1e-6) + 2 d before it, which the FP32 skin predicate accepts when
+// that squared, times (1 + 1e-6), is <= h2s; with sqrt(h2s) taken as
+// (1 + s) h (1 - 1e-6) the bound needs no square root.
+__device__ __forceinline__ double skin_move_limit(const Consts &c, float s) {
+    constexpr double kInvSqrt = 0.99999950000037499969;  // 1 / sqrt(1 + 1e-6)
+    const double h = (double)c.h;
+    return 0.5 * ((1.0 + (double)s) * h * (1.0 - 1e-6) * kInvSqrt - h * (1.0 + 1e-6)) * (1.0 - 1e-9);
+}
 
 // |b - a|^2 in FP64 from FP32 positions: the differences and squares are
 // exact, so the result is accurate to FP64 rounding.
@@ -214,6 +270,16 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
         if (__ballot(v >= t)) mx = t;
     }
     return mx;
+}
+
+// The scene's quantised largest move of the substep (the XSPH skin): the
+// largest of its kMoveSlots slots (uniform addresses: scalar loads).
+__device__ __forceinline__ uint32_t scene_move_q(const uint32_t *__restrict__ moves, uint32_t scene) {
+    const uint32_t *m = moves + (size_t)scene * kMoveSlots * kMoveStride;
+    uint32_t q = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < kMoveSlots; ++k) q = q > m[k * kMoveStride] ? q : m[k * kMoveStride];
+    return q;
 }
 
 // Lane-mask forms of per-lane predicates.  lanes(p) is the wave's mask of

@@ -30,16 +30,17 @@ uint32_t scan_tiles(uint32_t m);
 bool valid_block_size(int bs);
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
-                 uint32_t *counts, uint32_t *flags);
+                 uint32_t *counts, uint32_t *flags, uint32_t *moves);
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *err);
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *moves,
+                   uint32_t *err);
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
-                    uint32_t *flags);
+                    uint32_t *flags, const uint32_t *moves);
 // The neighbour passes take `flat`: walk the masks as one flattened stream
 // of set bits (where the rows fit) rather than row by row; results are the
 // same either way.  launch_normals processes the scenes without lazy_normal,
@@ -49,7 +50,7 @@ void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool
 void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
                    const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
                    const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout,
-                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags);
+                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
 void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout);

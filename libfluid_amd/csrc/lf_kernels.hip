@@ -48,13 +48,14 @@ namespace lf {
 __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                  const float4 *__restrict__ P, uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts,
-                                                 uint32_t *__restrict__ flags) {
+                                                 uint32_t *__restrict__ flags, uint32_t *__restrict__ moves) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
     const uint32_t i = slot0 + threadIdx.x;
     if (i == c.first) set_flag(flags, c.scene, kFragileByte, 0u);  // set by k_density
+    if (i - c.first < kMoveSlots) *move_slot(moves, c.scene, i - c.first) = 0u;  // raised by k_gather
     const int lane = threadIdx.x & 63;
     const bool active = i < c.first + c.count;
     const uint32_t b = active ? c.cell_base + bucket_of(c, P[i]) : 0xFFFFFFFFu;
@@ -191,33 +192,48 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
                                                    const float4 *__restrict__ L, float4 *__restrict__ Ps,
                                                    float4 *__restrict__ Vs, float4 *__restrict__ As,
                                                    float4 *__restrict__ Ls, float *__restrict__ soa, uint32_t nslots,
-                                                   uint32_t *__restrict__ err) {
+                                                   uint32_t *__restrict__ moves, uint32_t *__restrict__ err) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
     const uint32_t s = slot0 + threadIdx.x;
-    if (s >= cp->first + cp->count) return;
-    const uint32_t src = slot_src[s], id = slot_id[s];
-    const uint32_t b = keys[src];
-    const uint32_t shift = slot_shift(*cp, start);
-    const uint32_t lo = start[b] + shift, hi = start[b + 1] + shift;
-    uint32_t f;
-    if (hi - lo <= kRankFixMax) {
-        uint32_t r = 0;
-        for (uint32_t t = lo; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
-        f = lo + r;
-    } else {
-        f = s;  // bucket too large to rank: keep the provisional slot, report it
-        if (s == lo) atomicOr(err, 1u);
+    uint32_t q = 0u;
+    if (s < cp->first + cp->count) {
+        const uint32_t src = slot_src[s], id = slot_id[s];
+        const uint32_t b = keys[src];
+        const uint32_t shift = slot_shift(*cp, start);
+        const uint32_t lo = start[b] + shift, hi = start[b + 1] + shift;
+        uint32_t f;
+        if (hi - lo <= kRankFixMax) {
+            uint32_t r = 0;
+            for (uint32_t t = lo; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
+            f = lo + r;
+        } else {
+            f = s;  // bucket too large to rank: keep the provisional slot, report it
+            if (s == lo) atomicOr(err, 1u);
+        }
+        const float4 p = P[src], v = V[src], a = A[src];
+        Ps[f] = p;
+        soa[f] = p.x;
+        soa[nslots + f] = p.y;
+        soa[2 * (size_t)nslots + f] = p.z;
+        Vs[f] = v;
+        As[f] = a;
+        Ls[f] = L[src];
+        q = skin_quantum(*cp, v, a);
     }
-    const float4 p = P[src];
-    Ps[f] = p;
-    soa[f] = p.x;
-    soa[nslots + f] = p.y;
-    soa[2 * (size_t)nslots + f] = p.z;
-    Vs[f] = V[src];
-    As[f] = A[src];
-    Ls[f] = L[src];
+    // The scene's largest move of the substep (XSPH skin): reduced over the
+    // workgroup, then one atomicMax into the slot of the workgroup's index.
+    __shared__ uint32_t wq[kBlock / 64];
+    q = wave_max<10>(q);
+    if ((threadIdx.x & 63u) == 0u) wq[threadIdx.x >> 6] = q;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = wq[0];
+#pragma unroll
+        for (int w = 1; w < kBlock / 64; ++w) m = m > wq[w] ? m : wq[w];
+        atomicMax(move_slot(moves, cp->scene, blockIdx.x % kMoveSlots), m);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -397,11 +413,13 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
                                                 const float *__restrict__ soa, const uint32_t *__restrict__ start,
                                                 float4 *__restrict__ D,
                                                 float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
-                                                RowMasks MS, uint32_t *__restrict__ flags) {
+                                                RowMasks MS, uint32_t *__restrict__ flags,
+                                                const uint32_t *__restrict__ moves) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
+    const float h2s = skin_h2s(c, skin_frac(scene_move_q(moves, c.scene)));
     const uint32_t i = slot0 + threadIdx.x;
     if (i == c.first) set_flag(flags, c.scene, kFarMoveByte, 0u);  // set by k_forces when the skin is exceeded
     if (i >= c.first + c.count) return;
@@ -441,7 +459,7 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
             uint64_t live = lanes(k < len);
             if (r == 4) live &= ~lanes(k == kself);
-            const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= c.h2s) & live;
+            const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
             const float t = keep(c.h2 - rr, in);
             S += t * t * t;
             m = shift_in(m, in);
@@ -580,7 +598,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
             tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
             [&](const float4 &q, uint32_t, bool ok) { normal_term<F>(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     };
-    if (__builtin_expect(flag_set(flags[c.scene], kFragileByte), 0))
+    if (__builtin_expect(flag_set(flag_word(flags, c.scene), kFragileByte), 0))
         walk.template operator()<true>();
     else
         walk.template operator()<false>();
@@ -604,15 +622,18 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                                                RowMasks M, int flat, float4 *__restrict__ Pout,
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
                                                float4 *__restrict__ Lout, float4 *__restrict__ X,
-                                               float4 *__restrict__ Fdbg, uint32_t *__restrict__ flags) {
+                                               float4 *__restrict__ Fdbg, uint32_t *__restrict__ flags,
+                                               const uint32_t *__restrict__ moves) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
+    const uint32_t move_q = scene_move_q(moves, c.scene);
     const uint32_t i = slot0 + threadIdx.x;
     if (i >= c.first + c.count) return;
     const uint32_t *st = start + c.cell_base;
     const uint32_t shift = c.first - st[0];
+    const uint32_t sw = flag_word(flags, c.scene);
     const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
     const float4 di = D[i];
     float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
@@ -715,7 +736,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     walk.template operator()<false, false, F>();
             }
         };
-        if (__builtin_expect(flag_set(flags[c.scene], kFragileByte), 0))
+        if (flag_set(sw, kFragileByte))
             terms.template operator()<true>();
         else
             terms.template operator()<false>();
@@ -794,8 +815,9 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     // Move of the FP32 positions the pair predicates see (the differences
     // are exact in FP64).  Every writer stores the same value; NaN counts
     // as too far.
+    const double move_limit = skin_move_limit(c, skin_frac(move_q));
     const double ex = (double)hx - (double)pi.x, ey = (double)hy - (double)pi.y, ez = (double)hz - (double)pi.z;
-    if (!(sqrt(ex * ex + ey * ey + ez * ez) <= c.move_limit)) set_flag(flags, c.scene, kFarMoveByte, 1u);
+    if (!(ex * ex + ey * ey + ez * ez <= move_limit * move_limit)) set_flag(flags, c.scene, kFarMoveByte, 1u);
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
@@ -826,7 +848,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
         Vout[i] = vi;
         return;
     }
-    const uint32_t fl = flags[c.scene];
+    const uint32_t fl = flag_word(flags, c.scene);
     const bool masks_ok = c.use_masks && !flag_set(fl, kFarMoveByte);
     const float4 xi = X[i];
     const float rho_i = xi.w;
@@ -1119,8 +1141,8 @@ static inline void with_bs(int bs, F &&body) {
 bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
-                 uint32_t *counts, uint32_t *flags) {
-    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags);
+                 uint32_t *counts, uint32_t *flags, uint32_t *moves) {
+    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
@@ -1139,17 +1161,18 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *err) {
+                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *moves,
+                   uint32_t *err) {
     LF_GO(L, k_gather, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          A, Lw, Ps, Vs, As, Ls, soa, pv.nslots, err);
+          A, Lw, Ps, Vs, As, Ls, soa, pv.nslots, moves, err);
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
-                    uint32_t *flags) {
+                    uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
-              MS, flags);
+              MS, flags, moves);
     });
 }
 
@@ -1165,10 +1188,10 @@ void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const
                    const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
                    const float4 *N,
                    const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
-                   float4 *X, float4 *Fdbg, uint32_t *flags) {
+                   float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
-              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, flags);
+              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
     });
 }
 

@@ -160,7 +160,8 @@ class Pool {
     RowMasks M_{nullptr, nullptr}, MS_{nullptr, nullptr};
     uint32_t *keys_ = nullptr, *ranks_ = nullptr, *slot_src_ = nullptr, *slot_id_ = nullptr;
     uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr, *err_ = nullptr;
-    uint32_t *flags_ = nullptr;  // per scene: the substep's far-move and near-isolated flags, a byte each
+    uint32_t *flags_ = nullptr;  // kSceneWords per scene: the substep's flag bytes
+    uint32_t *moves_ = nullptr;  // kMoveSlots * kMoveStride per scene: the substep's largest move
     Consts *d_cs_ = nullptr;
     uint32_t *d_chunk_ = nullptr;
 

@@ -28,18 +28,6 @@ namespace {
 
 constexpr double kPi = 3.141592654;  // CUDART_PI_F as defined by the reference (fluiddefines.h:16)
 
-// XSPH skin radius (1 + skin) h.  The integrator moves a particle by
-// v dt + a dt^2 / 2 with |a| <= acc_limit and |v| at most about vel_limit
-// (velocities are clamped to it before XSPH blends them), so the skin is
-// sized for both particles of a pair moving that far, with 10% to spare,
-// between 0.02 and 0.1.  A larger move sets the scene's far-move flag and
-// the XSPH pass scans (the result is the same; only the speed differs).
-double skin_of(const lf_params &p, double dt) {
-    const double move = (double)p.vel_limit * dt + 0.5 * (double)p.acc_limit * dt * dt;
-    const double s = 2.2 * move / (double)p.h;
-    return (s > 0.0 && s < 0.1) ? (s > 0.02 ? s : 0.02) : 0.1;
-}
-
 int check_launch(const char *what) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(LF_ERR_HIP, "%s: kernel launch failed: %s", what, hipGetErrorString(e));
@@ -70,12 +58,7 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.h2 = p.h * p.h;  // FP32 product, the pair-predicate threshold (m_h2)
     c.inv_h = (float)(1.0 / h);
     c.eps = FLT_MIN;
-    const double skin = skin_of(p, dt);
-    c.h2s = (float)((1.0 + skin) * (1.0 + skin) * (double)c.h2);
-    // A pair within h after the move (FP32 predicate: true distance below
-    // h (1 + 1e-6)) was within h (1 + 1e-6) + 2 d before it, which the FP32
-    // skin predicate accepts when that squared, times (1 + 1e-6), is <= h2s.
-    c.move_limit = 0.5 * (std::sqrt((double)c.h2s / (1.0 + 1e-6)) - (double)c.h * (1.0 + 1e-6)) * (1.0 - 1e-9);
+    c.skin_q = (float)(2.2 * 4096.0 / h);  // the substep's skin is sized on the device from its largest move
     c.use_masks = (debug_flags & LF_DEBUG_SCAN) ? 0 : 1;
     c.mC6 = (float)(m * C6);
     c.rho0 = p.rest_density;
@@ -132,8 +115,8 @@ void Pool::release() {
     stop_exporter();
     drop_graph();
     void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, soa_, M_.w32, M_.w64, MS_.w32,
-                    MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, flags_, d_cs_,
-                    d_chunk_};
+                    MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, flags_, moves_,
+                    d_cs_, d_chunk_};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &r : prof_live_) {
@@ -194,7 +177,8 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (!rc) rc = dalloc(&start_, (size_t)nbuckets_ + 1);
     if (!rc) rc = dalloc(&bsum_, (size_t)ntiles_ + 1);
     if (!rc) rc = dalloc(&err_, 1);
-    if (!rc) rc = dalloc(&flags_, scene_cap);
+    if (!rc) rc = dalloc(&flags_, (size_t)kSceneWords * scene_cap);
+    if (!rc) rc = dalloc(&moves_, (size_t)kMoveSlots * kMoveStride * scene_cap);
     if (!rc) rc = dalloc(&d_cs_, scene_cap);
     if (!rc) rc = dalloc(&d_chunk_, h_chunk_.size());
     for (int k = 0; k < kTableRing && !rc; ++k) {
@@ -209,7 +193,9 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         e = hipMemsetAsync(counts_, 0, sizeof(uint32_t) * nbuckets_, stream_);
         if (e == hipSuccess) e = hipMemsetAsync(start_, 0, sizeof(uint32_t) * (nbuckets_ + 1), stream_);
         if (e == hipSuccess) e = hipMemsetAsync(err_, 0, sizeof(uint32_t), stream_);
-        if (e == hipSuccess) e = hipMemsetAsync(flags_, 0, sizeof(uint32_t) * scene_cap, stream_);
+        if (e == hipSuccess) e = hipMemsetAsync(flags_, 0, sizeof(uint32_t) * kSceneWords * scene_cap, stream_);
+        if (e == hipSuccess)
+            e = hipMemsetAsync(moves_, 0, sizeof(uint32_t) * kMoveSlots * kMoveStride * scene_cap, stream_);
         if (e != hipSuccess) rc = fail(LF_ERR_HIP, "pool: %s", hipGetErrorString(e));
     }
     if (!rc) rc = push_tables();
@@ -485,11 +471,12 @@ bool Pool::any_separate_normals() const {
 // A0, L0) into cell order (P1, V1, A1, L1).
 void Pool::enqueue_sort() {
     const PoolView pv = view();
-    timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_); });
+    timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_); });
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, nbuckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
-        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, L0_, P1_, V1_, A1_, L1_, soa_, err_);
+        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, L0_, P1_, V1_, A1_, L1_, soa_, moves_,
+                      err_);
     });
 }
 
@@ -500,7 +487,7 @@ void Pool::enqueue_substep() {
     const PoolView pv = view();
     enqueue_sort();
     timed(LF_K_DENSITY, [&](const Launch &L) {
-        launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, flags_);
+        launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, flags_, moves_);
     });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
@@ -508,7 +495,7 @@ void Pool::enqueue_substep() {
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
         launch_forces(L, pv, tune_.forces, tune_.flat_forces != 0, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
-                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_);
+                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
         launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph != 0, P1_, X_, V2_, L0_, start_, MS_, flags_, V0_);

@@ -324,8 +324,10 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
     """Walking the density pass's neighbour masks, as one flattened stream or
     row by row, gives bitwise the same trajectory as testing every stencil
     slot (same neighbours, same order).  c1 moves little per substep (XSPH
-    walks the skin masks); the double dam's dt = 2 ms moves particles farther
-    than the skin allows (XSPH scans); the dense block has stencil rows longer
+    walks the skin masks, sized from each substep's largest move), except in
+    its first, where the wall projection moves the particles that start on
+    the walls farther than that skin allows (XSPH scans); the double dam's
+    dt = 2 ms moves particles farther than the largest skin allows; the dense block has stencil rows longer
     than 64 slots (those rows are scanned in every pass, and their waves walk
     row by row); the packed block (about 12 particles a cell) has rows of 33
     to 64 slots, which the flattened walk splits into two 32-slot parts."""
