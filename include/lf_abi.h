@@ -207,15 +207,17 @@ int lf_set_tuning(lf_scene *s, const int block_sizes[4]);
 int lf_get_tuning(const lf_scene *s, int block_sizes[4]);
 /* Walk modes of the normals, forces and XSPH passes: 1 walks the density
  * pass's neighbour masks as one flattened stream of set bits (in waves whose
- * stencil rows all fit 64-slot masks), 0 walks them row by row.  Default
- * {1, 1, 0} (measured fastest for the force walk's arithmetic-heavy steps and
- * the XSPH walk's gather-heavy ones); lf_autotune picks per pass.  Results do
- * not depend on them. */
+ * stencil rows all fit 64-slot masks), 0 walks them row by row, 2 flattens
+ * only the waves in which a row holds more than 32 slots.  Default {1, 1, 2}
+ * (measured fastest: the force walk's arithmetic-heavy steps gain from the
+ * flattened stream everywhere, the XSPH walk's gather-heavy ones only where
+ * rows are long); lf_autotune picks per pass.  Results do not depend on
+ * them. */
 int lf_set_walks(lf_scene *s, const int flat[3]);
 int lf_get_walks(const lf_scene *s, int flat[3]);
 /* Times each block size on the scene's current state (`reps` substeps each,
  * state restored) and keeps the fastest per pass; then, at those block
- * sizes, both walk modes, keeping the faster per pass.  ms (may be NULL)
+ * sizes, the three walk modes, keeping the fastest per pass.  ms (may be NULL)
  * receives the 12 block-size averages: pass-major, block sizes 128, 256, 512.
  * Needs one lf_step first (the substep length). */
 int lf_autotune(lf_scene *s, uint32_t reps, double *ms);

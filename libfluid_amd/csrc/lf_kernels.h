@@ -41,17 +41,18 @@ void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
                     uint32_t *flags, const uint32_t *moves);
-// The neighbour passes take `flat`: walk the masks as one flattened stream
-// of set bits (where the rows fit) rather than row by row; results are the
-// same either way.  launch_normals processes the scenes without lazy_normal,
+// The neighbour passes take `flat`: 1 walks the masks as one flattened
+// stream of set bits (where the rows fit) rather than row by row (0), 2 does
+// so only in waves with a row of more than 32 slots; results are the same
+// either way.  launch_normals processes the scenes without lazy_normal,
 // or with `all` every scene (the on-demand normals of the last substep).
-void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool all, const float4 *Q,
+void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool all, const float4 *Q,
                     const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N);
-void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
+void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
                    const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
                    const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout,
                    float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
-void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *X,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout);
 

@@ -317,13 +317,22 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 // ahead, so a row change does not wait on LDS.  Visits the same slots in the
 // same order as walk_masks.  Returns false, having visited nothing, when a
 // lane of the wave has a row of more than 64 slots or more than kFlatEntries
-// non-empty pseudo-rows.
+// non-empty pseudo-rows, or (need_long) when no lane of the wave has a row
+// of more than 32 slots (one pseudo-row per row: the row-by-row walk then
+// does as well without the table).
 constexpr int kFlatEntries = 16;
 constexpr int kForceWalkWidth = 4;  // set bits per step of the flattened force walk
 template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
-                                          const RowMasks &mask, uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
+                                          const RowMasks &mask, uint32_t nslots, uint32_t i, bool need_long, Ld &&ld,
+                                          Fn &&fn) {
     const uint32_t t = threadIdx.x;
+    if (need_long) {
+        bool lng = false;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) lng = lng || e[r] - b[r] > 32u;
+        if (!__ballot(lng)) return false;
+    }
     // Each row's mask with one unconditional 8-byte load (from the 64-bit
     // array for a row of 33..64 slots, else from the 32-bit array, whose
     // allocation is padded so that the word after the last entry is
@@ -395,9 +404,9 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
 // choice is wave-uniform and does not change the result.
 template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
-                                          const RowMasks &mask, bool use, bool flat, uint32_t nslots, uint32_t i,
+                                          const RowMasks &mask, bool use, int flat, uint32_t nslots, uint32_t i,
                                           Ld &&ld, Fn &&fn) {
-    if (use && flat && walk_flat<BS, W>(tab, b, e, mask, nslots, i, ld, fn)) return;
+    if (use && flat && walk_flat<BS, W>(tab, b, e, mask, nslots, i, flat == 2, ld, fn)) return;
     walk_masks(b, e, mask, use, nslots, i, ld, fn);
 }
 
@@ -595,7 +604,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
         walk_rows<BS, 2>(
-            tab, b, e, M, c.use_masks, flat != 0, nslots, i, [&](uint32_t j) { return at32(Q, j); },
+            tab, b, e, M, c.use_masks, flat, nslots, i, [&](uint32_t j) { return at32(Q, j); },
             [&](const float4 &q, uint32_t, bool ok) { normal_term<F>(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     };
     if (__builtin_expect(flag_set(flag_word(flags, c.scene), kFragileByte), 0))
@@ -661,7 +670,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 float4 v, d, n;
             };
             walk_rows<BS, kForceWalkWidth>(
-                tab, b, e, M, c.use_masks, flat != 0, nslots, i,
+                tab, b, e, M, c.use_masks, flat, nslots, i,
                 [&](uint32_t j) {
                     Cand cd;
                     cd.q = at32(Q, j);
@@ -866,7 +875,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
         walk_rows<BS, 2>(
-            tab, b, e, MS, masks_ok, flat != 0, nslots, i,
+            tab, b, e, MS, masks_ok, flat, nslots, i,
             [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
             [&](const Cand &cj, uint32_t j, bool ok) {
                 const float4 q = cj.x;
@@ -1176,31 +1185,31 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
     });
 }
 
-void launch_normals(const Launch &L, const PoolView &pv, int bs, bool flat, bool all, const float4 *Q,
+void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool all, const float4 *Q,
                     const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat ? 1 : 0,
+        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat,
               all ? 1 : 0, flags, N);
     });
 }
 
-void launch_forces(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *Q,
+void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
                    const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
                    const float4 *N,
                    const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
                    float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
-              start, M, flat ? 1 : 0, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
+              start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
     });
 }
 
-void launch_xsph(const Launch &L, const PoolView &pv, int bs, bool flat, const float4 *Ps, const float4 *X,
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *X,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
-              flat ? 1 : 0, flags, Vout);
+              flat, flags, Vout);
     });
 }
 

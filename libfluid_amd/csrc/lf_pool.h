@@ -25,7 +25,7 @@ namespace lf {
 // (1) or row by row (0).  Results do not depend on either.
 struct Tuning {
     int density = 256, normals = 256, forces = 256, xsph = 256;
-    int flat_normals = 1, flat_forces = 1, flat_xsph = 0;
+    int flat_normals = 1, flat_forces = 1, flat_xsph = 2;
 };
 
 struct SceneRec {

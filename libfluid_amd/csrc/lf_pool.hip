@@ -491,14 +491,14 @@ void Pool::enqueue_substep() {
     });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
-            launch_normals(L, pv, tune_.normals, tune_.flat_normals != 0, false, Q_, start_, M_, flags_, N_);
+            launch_normals(L, pv, tune_.normals, tune_.flat_normals, false, Q_, start_, M_, flags_, N_);
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
-        launch_forces(L, pv, tune_.forces, tune_.flat_forces != 0, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
+        launch_forces(L, pv, tune_.forces, tune_.flat_forces, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
                       P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
-        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph != 0, P1_, X_, V2_, L0_, start_, MS_, flags_, V0_);
+        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, X_, V2_, L0_, start_, MS_, flags_, V0_);
     });
 }
 
@@ -705,7 +705,7 @@ int Pool::get_field(uint32_t id, int field, void *dst, size_t bytes) {
             if (!s->fields)
                 return fail(LF_ERR_STATE, "normals: no substep data for this scene (none since it was added, "
                                           "loaded, moved, re-parametrised or paused, or autotuning ran since)");
-            launch_normals(Launch{stream_, nullptr, nullptr}, view(), tune_.normals, tune_.flat_normals != 0, true,
+            launch_normals(Launch{stream_, nullptr, nullptr}, view(), tune_.normals, tune_.flat_normals, true,
                            Q_, start_, M_, flags_, N_);
             if (int rc = check_launch("normals")) return rc;
             LF_HIP(hipStreamSynchronize(stream_));
@@ -870,8 +870,11 @@ int Pool::set_tuning(const Tuning &t) {
     if (!valid_block_size(t.density) || !valid_block_size(t.normals) || !valid_block_size(t.forces) ||
         !valid_block_size(t.xsph))
         return fail(LF_ERR_INVALID, "tuning: block sizes must be 128, 256 or 512");
-    if ((t.flat_normals | t.flat_forces | t.flat_xsph) & ~1)
-        return fail(LF_ERR_INVALID, "tuning: walk modes must be 0 (row by row) or 1 (flattened)");
+    for (int f : {t.flat_normals, t.flat_forces, t.flat_xsph})
+        if (f < 0 || f > 2)
+            return fail(LF_ERR_INVALID,
+                        "tuning: walk modes must be 0 (row by row), 1 (flattened) or 2 (flattened in waves with a "
+                        "row of more than 32 slots)");
     tune_ = t;
     drop_graph();
     return LF_OK;
@@ -941,14 +944,17 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     chosen.normals = sizes[best[1]];
     chosen.forces = sizes[best[2]];
     chosen.xsph = sizes[best[3]];
-    double walk[2][4] = {};  // [row by row, flattened][pass]
-    for (int f = 0; f < 2 && !rc; ++f) {
+    double walk[3][4] = {};  // [row by row, flattened, flattened in waves with long rows][pass]
+    for (int f = 0; f < 3 && !rc; ++f) {
         Tuning t = chosen;
         t.flat_normals = t.flat_forces = t.flat_xsph = f;
         time_shape(t, walk[f]);
     }
-    auto pick = [&](int k) {  // flattened unless row by row was measurably faster
-        return (walk[0][k] > 0.0 && walk[0][k] < walk[1][k]) ? 0 : 1;
+    auto pick = [&](int k) {  // flattened unless another mode was measurably faster
+        int m = 1;
+        for (int f : {2, 0})
+            if (walk[f][k] > 0.0 && walk[f][k] < walk[m][k]) m = f;
+        return m;
     };
     chosen.flat_normals = pick(1);
     chosen.flat_forces = pick(2);

@@ -209,7 +209,8 @@ class Scene:
 
     def set_walks(self, flat) -> None:
         """Walk modes of the normals, forces and XSPH passes (1 flattened,
-        0 row by row)."""
+        0 row by row, 2 flattened in waves with a row of more than 32
+        slots)."""
         f = (C.c_int * 3)(*[int(x) for x in flat])
         check(lib.lf_set_walks(self._h, f), "lf_set_walks")
 
@@ -219,7 +220,7 @@ class Scene:
         return tuple(f)
 
     def autotune(self, reps: int = 5) -> np.ndarray:
-        """Keeps the fastest block size per pass, then the faster walk mode per
+        """Keeps the fastest block size per pass, then the fastest walk mode per
         pass; returns the (4, 3) ms table of the block-size sweep (passes
         density/normals/forces/xsph x block sizes 128/256/512)."""
         ms = (C.c_double * 12)()

@@ -321,13 +321,15 @@ def test_full_size_spray_c3(gpu, oracle_omp):
 
 @pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed"])
 def test_neighbour_masks_are_bitwise_transparent(gpu, case):
-    """Walking the density pass's neighbour masks, as one flattened stream or
-    row by row, gives bitwise the same trajectory as testing every stencil
-    slot (same neighbours, same order).  c1 moves little per substep (XSPH
+    """Walking the density pass's neighbour masks, as one flattened stream,
+    row by row, or flattened only in waves with rows of more than 32 slots,
+    gives bitwise the same trajectory as testing every stencil slot (same
+    neighbours, same order).  c1 moves little per substep (XSPH
     walks the skin masks, sized from each substep's largest move), except in
     its first, where the wall projection moves the particles that start on
     the walls farther than that skin allows (XSPH scans); the double dam's
-    dt = 2 ms moves particles farther than the largest skin allows; the dense block has stencil rows longer
+    dt = 2 ms moves particles farther than the largest skin allows; the
+    dense block has stencil rows longer
     than 64 slots (those rows are scanned in every pass, and their waves walk
     row by row); the packed block (about 12 particles a cell) has rows of 33
     to 64 slots, which the flattened walk splits into two 32-slot parts."""
@@ -335,7 +337,7 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
            "dense": configs.dense_block,
            "packed": lambda: configs.dense_block(24, 24, 24, spacing=configs.H / 2.3)}[case]()
     outs = []
-    for scan, flat in ((True, 1), (False, 1), (False, 0)):
+    for scan, flat in ((True, 1), (False, 1), (False, 0), (False, 2)):
         sc = gpu.Scene(cfg.params, cfg.n)
         try:
             sc.upload(cfg.pos, cfg.vel)
@@ -346,7 +348,8 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
             outs.append(sc.download(L.LF_ORDER_ID))
         finally:
             sc.close()
-    for m, name in ((1, "flattened mask walks"), (2, "row-by-row mask walks")):
+    for m, name in ((1, "flattened mask walks"), (2, "row-by-row mask walks"),
+                    (3, "mask walks flattened where rows are long")):
         for k in ("pos", "vel", "acc", "colour"):
             assert np.array_equal(outs[0][k], outs[m][k]), f"{case}: {k} differs between {name} and full scans"
 
