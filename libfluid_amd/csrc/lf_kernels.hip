@@ -322,6 +322,7 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 // does as well without the table).
 constexpr int kFlatEntries = 16;
 constexpr int kForceWalkWidth = 4;  // set bits per step of the flattened force walk
+constexpr int kXsphWalkWidth = 4;   // and of the flattened XSPH walk
 template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, uint32_t nslots, uint32_t i, bool need_long, Ld &&ld,
@@ -874,7 +875,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     };
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
-        walk_rows<BS, 2>(
+        walk_rows<BS, kXsphWalkWidth>(
             tab, b, e, MS, masks_ok, flat, nslots, i,
             [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
             [&](const Cand &cj, uint32_t j, bool ok) {
