@@ -34,10 +34,12 @@ void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t 
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);
+// Copies positions and velocities to cell order; order[f] receives the state
+// slot of the particle in slot f (acceleration and low position words stay
+// in the state arrays, read there by the force pass).
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *moves,
-                   uint32_t *err);
+                   float4 *Ps, float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err);
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
                     uint32_t *flags, const uint32_t *moves);
@@ -48,10 +50,12 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
 // or with `all` every scene (the on-demand normals of the last substep).
 void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool all, const float4 *Q,
                     const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N);
+// A and Lw are the state arrays, read at order[i]; Aout and Lout must be
+// other buffers (they are written in cell order).
 void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
-                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
-                   const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout,
-                   float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
+                   const float *Pr, const float4 *Vs, const float4 *A, const float4 *Lw, const uint32_t *order,
+                   const float4 *D, const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout,
+                   float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
 void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *X,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout);
@@ -67,6 +71,8 @@ void launch_to_id_order(hipStream_t st, uint32_t first, uint32_t count, const fl
 void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, const float4 *P, const float4 *V,
                  const float4 *A, const float4 *Lw, const float4 *D, float4 *P2, float4 *V2, float4 *A2, float4 *L2,
                  float4 *D2);
+void launch_carry(hipStream_t st, uint32_t first, uint32_t count, const float4 *A, const float4 *Lw, float4 *A2,
+                  float4 *L2);
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,
                    const float4 *D, bool by_id, float *out, uint32_t *ids);
 // Exact state (12 floats per particle: P.xyz, L.xyz, V.xyz, A.xyz), id order.

@@ -182,17 +182,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Consts *__restrict__ c
 // 4. stable placement + payload gather (sort_by_key of Points, Velocity,
 // Acceleration, fluidsystem.cu:309-313).  The final slot of a particle is
 // its bucket start plus the number of bucket members with a smaller id, so
-// the order is (key, id) whatever order the atomics handed out.
+// the order is (key, id) whatever order the atomics handed out.  Position
+// and velocity are copied to cell order (the neighbour passes gather them);
+// the acceleration and the position's low word, which only the particle's
+// own integration reads, stay where they are: order[f] records the state
+// slot the particle in slot f came from.
 __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                    const uint32_t *__restrict__ slot_src,
                                                    const uint32_t *__restrict__ slot_id,
                                                    const uint32_t *__restrict__ keys,
                                                    const uint32_t *__restrict__ start, const float4 *__restrict__ P,
                                                    const float4 *__restrict__ V, const float4 *__restrict__ A,
-                                                   const float4 *__restrict__ L, float4 *__restrict__ Ps,
-                                                   float4 *__restrict__ Vs, float4 *__restrict__ As,
-                                                   float4 *__restrict__ Ls, float *__restrict__ soa, uint32_t nslots,
-                                                   uint32_t *__restrict__ moves, uint32_t *__restrict__ err) {
+                                                   float4 *__restrict__ Ps, float4 *__restrict__ Vs,
+                                                   uint32_t *__restrict__ order, float *__restrict__ soa,
+                                                   uint32_t nslots, uint32_t *__restrict__ moves,
+                                                   uint32_t *__restrict__ err) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
@@ -218,8 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
         soa[nslots + f] = p.y;
         soa[2 * (size_t)nslots + f] = p.z;
         Vs[f] = v;
-        As[f] = a;
-        Ls[f] = L[src];
+        order[f] = src;
         q = skin_quantum(*cp, v, a);
     }
     // The scene's largest move of the substep (XSPH skin): reduced over the
@@ -626,8 +629,9 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                uint32_t nslots, const float4 *__restrict__ Ps,
                                                const float4 *__restrict__ Q, const float *__restrict__ Pr,
-                                               const float4 *__restrict__ Vs, const float4 *__restrict__ As,
-                                               const float4 *__restrict__ Ls, const float4 *__restrict__ D,
+                                               const float4 *__restrict__ Vs, const float4 *__restrict__ A,
+                                               const float4 *__restrict__ Lw, const uint32_t *__restrict__ order,
+                                               const float4 *__restrict__ D,
                                                const float4 *__restrict__ N, const uint32_t *__restrict__ start,
                                                RowMasks M, int flat, float4 *__restrict__ Pout,
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
@@ -644,7 +648,8 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     const uint32_t *st = start + c.cell_base;
     const uint32_t shift = c.first - st[0];
     const uint32_t sw = flag_word(flags, c.scene);
-    const float4 pi = Ps[i], vi = Vs[i], ai = As[i];
+    const uint32_t src = order[i];  // the particle's slot in the state arrays (A, Lw)
+    const float4 pi = Ps[i], vi = Vs[i];
     const float4 di = D[i];
     float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
     if (i < st[c.ncells] + shift) {
@@ -788,7 +793,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
         ay *= k;
         az *= k;
     }
-    const float4 li = Ls[i];
+    const float4 ai = A[src], li = Lw[src];
     double x = ((double)pi.x + (double)li.x) + ((double)vi.x * c.dt + 0.5 * (double)ai.x * c.dtdt);
     double y = ((double)pi.y + (double)li.y) + ((double)vi.y * c.dt + 0.5 * (double)ai.y * c.dtdt);
     double z = ((double)pi.z + (double)li.z) + ((double)vi.z * c.dt + 0.5 * (double)ai.z * c.dtdt);
@@ -975,6 +980,18 @@ __global__ __launch_bounds__(kBlock) void k_move(uint32_t from, uint32_t to, uin
     A2[to + k] = A[from + k];
     L2[to + k] = L[from + k];
     D2[to + k] = D[from + k];
+}
+
+// Copies `count` slots from `first` of (A, L) to (A2, L2): a paused scene's
+// acceleration and low position words, carried to the buffers that become
+// the state after a substep (the substep's kernels skip its slots).
+__global__ __launch_bounds__(kBlock) void k_carry(uint32_t first, uint32_t count, const float4 *__restrict__ A,
+                                                  const float4 *__restrict__ L, float4 *__restrict__ A2,
+                                                  float4 *__restrict__ L2) {
+    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    A2[first + k] = A[first + k];
+    L2[first + k] = L[first + k];
 }
 
 // AoS staging for downloads: 12 floats per particle (pos, colour, vel, acc).
@@ -1171,10 +1188,9 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   const float4 *Lw, float4 *Ps, float4 *Vs, float4 *As, float4 *Ls, float *soa, uint32_t *moves,
-                   uint32_t *err) {
+                   float4 *Ps, float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
     LF_GO(L, k_gather, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          A, Lw, Ps, Vs, As, Ls, soa, pv.nslots, moves, err);
+          A, Ps, Vs, order, soa, pv.nslots, moves, err);
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
@@ -1195,13 +1211,12 @@ void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool 
 }
 
 void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
-                   const float *Pr, const float4 *Vs, const float4 *As, const float4 *Ls, const float4 *D,
-                   const float4 *N,
-                   const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout, float4 *Aout, float4 *Lout,
-                   float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
+                   const float *Pr, const float4 *Vs, const float4 *A, const float4 *Lw, const uint32_t *order,
+                   const float4 *D, const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout,
+                   float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, As, Ls, D, N,
-              start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
+        LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, A, Lw, order,
+              D, N, start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
     });
 }
 
@@ -1241,6 +1256,12 @@ void launch_move(hipStream_t st, uint32_t from, uint32_t to, uint32_t count, con
     if (count)
         hipLaunchKernelGGL(k_move, dim3(blocks_for(count)), dim3(kBlock), 0, st, from, to, count, P, V, A, Lw, D, P2,
                            V2, A2, L2, D2);
+}
+
+void launch_carry(hipStream_t st, uint32_t first, uint32_t count, const float4 *A, const float4 *Lw, float4 *A2,
+                  float4 *L2) {
+    if (count)
+        hipLaunchKernelGGL(k_carry, dim3(blocks_for(count)), dim3(kBlock), 0, st, first, count, A, Lw, A2, L2);
 }
 
 void launch_export(hipStream_t st, const Consts &c, const float4 *P, const float4 *V, const float4 *A,

@@ -137,6 +137,8 @@ class Pool {
     bool any_separate_normals() const;
     void enqueue_sort();
     void enqueue_substep();
+    void advance_state();
+    std::vector<std::pair<uint32_t, uint32_t>> paused_ranges() const;
     int run_substeps(uint32_t substeps);
     hipEvent_t take_event();
     template <class Fn>
@@ -179,10 +181,12 @@ class Pool {
     bool tables_ev_pending_[kTableRing] = {};
     int table_slot_ = 0;
 
-    // captured substep
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t graph_exec_ = nullptr;
-    bool graph_normals_ = false;  // the captured substep has the normals pass
+    // captured substep, one per pairing of the state and scratch A/L buffers
+    hipGraph_t graph_[2] = {};
+    hipGraphExec_t graph_exec_[2] = {};
+    int parity_ = 0;              // which pairing is current (toggled by advance_state)
+    bool graph_normals_ = false;  // the captured substeps have the normals pass
+    std::vector<std::pair<uint32_t, uint32_t>> graph_paused_;  // and carry these paused scenes' slots
 
     Exporter *exporter_ = nullptr;
 

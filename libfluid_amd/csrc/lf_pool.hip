@@ -467,22 +467,25 @@ bool Pool::any_separate_normals() const {
     return false;
 }
 
-// Hash, scan, provisional scatter and stable gather of the state (P0, V0,
-// A0, L0) into cell order (P1, V1, A1, L1).
+// Hash, scan, provisional scatter and stable gather of the state positions
+// and velocities (P0, V0) into cell order (P1, V1); ranks_ then holds, per
+// cell-order slot, the particle's state slot (the force pass reads A0 and L0
+// there).
 void Pool::enqueue_sort() {
     const PoolView pv = view();
     timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_); });
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, nbuckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
-        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, L0_, P1_, V1_, A1_, L1_, soa_, moves_,
-                      err_);
+        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, P1_, V1_, ranks_, soa_, moves_, err_);
     });
 }
 
 // One substep of every active scene.  Buffer roles: state (P0,V0,A0,L0) ->
-// sorted (P1,V1,A1,L1) -> integrated P0, A0, L0 and pre-XSPH velocity V2 ->
-// XSPH velocity V0.
+// sorted (P1,V1) -> integrated P0, A1, L1 (cell order; A0 and L0 read
+// through ranks_) and pre-XSPH velocity V2 -> XSPH velocity V0.  The new
+// acceleration and low position words are in A1 and L1: advance_state()
+// makes them the state.
 void Pool::enqueue_substep() {
     const PoolView pv = view();
     enqueue_sort();
@@ -494,22 +497,42 @@ void Pool::enqueue_substep() {
             launch_normals(L, pv, tune_.normals, tune_.flat_normals, false, Q_, start_, M_, flags_, N_);
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
-        launch_forces(L, pv, tune_.forces, tune_.flat_forces, P1_, Q_, Pr_, V1_, A1_, L1_, D_, N_, start_, M_,
-                      P0_, V2_, A0_, L0_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
+        launch_forces(L, pv, tune_.forces, tune_.flat_forces, P1_, Q_, Pr_, V1_, A0_, L0_, ranks_, D_, N_, start_,
+                      M_, P0_, V2_, A1_, L1_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
-        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, X_, V2_, L0_, start_, MS_, flags_, V0_);
+        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, X_, V2_, L1_, start_, MS_, flags_, V0_);
     });
+    for (const auto &r : paused_ranges()) launch_carry(stream_, r.first, r.second, A0_, L0_, A1_, L1_);
 }
 
-// Launches of the executable graph may still be queued: wait for them before
-// destroying it.
+// Slot ranges of the live scenes that are paused (their A and L state is
+// carried across each substep's buffer swap).
+std::vector<std::pair<uint32_t, uint32_t>> Pool::paused_ranges() const {
+    std::vector<std::pair<uint32_t, uint32_t>> r;
+    for (const SceneRec &s : scenes_)
+        if (s.live && !s.active && s.n) r.emplace_back(s.first, s.n);
+    return r;
+}
+
+// After an enqueued substep: its acceleration and low position words (A1,
+// L1) become the state; the old ones are scratch until the next substep.
+void Pool::advance_state() {
+    std::swap(A0_, A1_);
+    std::swap(L0_, L1_);
+    parity_ ^= 1;
+}
+
+// Launches of the executable graphs may still be queued: wait for them
+// before destroying them.
 void Pool::drop_graph() {
-    if (graph_exec_ && stream_) (void)hipStreamSynchronize(stream_);
-    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
-    graph_exec_ = nullptr;
-    graph_ = nullptr;
+    if ((graph_exec_[0] || graph_exec_[1]) && stream_) (void)hipStreamSynchronize(stream_);
+    for (int p = 0; p < 2; ++p) {
+        if (graph_exec_[p]) (void)hipGraphExecDestroy(graph_exec_[p]);
+        if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
+        graph_exec_[p] = nullptr;
+        graph_[p] = nullptr;
+    }
 }
 
 void Pool::set_graphs(bool on) {
@@ -520,28 +543,39 @@ void Pool::set_graphs(bool on) {
 int Pool::run_substeps(uint32_t substeps) {
     if (int rc = push_tables()) return rc;
     if (!use_graphs_ || prof_mask_) {
-        for (uint32_t k = 0; k < substeps; ++k) enqueue_substep();
+        for (uint32_t k = 0; k < substeps; ++k) {
+            enqueue_substep();
+            advance_state();
+        }
         return check_launch("step");
     }
-    if (graph_exec_ && graph_normals_ != any_separate_normals()) drop_graph();
-    if (!graph_exec_) {
-        // One substep captured once; the scene table it reads lives in device
-        // memory, so later table changes need no new capture.
-        LF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-        enqueue_substep();
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(stream_, &g);
-        if (e != hipSuccess) return fail(LF_ERR_HIP, "step: graph capture failed: %s", hipGetErrorString(e));
-        e = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            graph_exec_ = nullptr;
-            return fail(LF_ERR_HIP, "step: graph instantiation failed: %s", hipGetErrorString(e));
+    if ((graph_exec_[0] || graph_exec_[1]) &&
+        (graph_normals_ != any_separate_normals() || graph_paused_ != paused_ranges()))
+        drop_graph();
+    for (uint32_t k = 0; k < substeps; ++k) {
+        hipGraphExec_t &ge = graph_exec_[parity_];
+        if (!ge) {
+            // One substep captured once per pairing of the state and scratch
+            // A/L buffers; the scene table it reads lives in device memory,
+            // so later table changes need no new capture.
+            LF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            enqueue_substep();
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(stream_, &g);
+            if (e != hipSuccess) return fail(LF_ERR_HIP, "step: graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                ge = nullptr;
+                return fail(LF_ERR_HIP, "step: graph instantiation failed: %s", hipGetErrorString(e));
+            }
+            graph_[parity_] = g;
+            graph_normals_ = any_separate_normals();
+            graph_paused_ = paused_ranges();
         }
-        graph_ = g;
-        graph_normals_ = any_separate_normals();
+        LF_HIP(hipGraphLaunch(ge, stream_));
+        advance_state();
     }
-    for (uint32_t k = 0; k < substeps; ++k) LF_HIP(hipGraphLaunch(graph_exec_, stream_));
     return LF_OK;
 }
 
@@ -754,6 +788,9 @@ int Pool::neighbours(uint32_t id, uint32_t *counts, uint32_t *list, uint64_t cap
     prof_mask_ = 0;
     enqueue_sort();
     prof_mask_ = keep_mask;
+    // The sort replaced the last substep's cell table and flags, which the
+    // normals computed when read would use.
+    for (auto &r : scenes_) r.fields = false;
     const size_t n = s->n;
     uint32_t *d_cnt = nullptr;
     if (int rc = dalloc(&d_cnt, n)) return rc;
@@ -895,9 +932,10 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     const size_t n = nslots_;
     float4 *save = nullptr;
     if (int rc = dalloc(&save, 5 * n)) return rc;
-    float4 *const state[5] = {P0_, V0_, A0_, L0_, D_};
+    // The state arrays as they are now (substeps swap the A and L buffers).
+    auto state = [&](int k) { return k == 0 ? P0_ : k == 1 ? V0_ : k == 2 ? A0_ : k == 3 ? L0_ : D_; };
     for (int k = 0; k < 5; ++k)
-        LF_HIP(hipMemcpyAsync(save + k * n, state[k], 16 * n, hipMemcpyDeviceToDevice, stream_));
+        LF_HIP(hipMemcpyAsync(save + k * n, state(k), 16 * n, hipMemcpyDeviceToDevice, stream_));
     std::vector<SceneRec> keep = scenes_;
     const Tuning before = tune_;
     const uint32_t keep_mask = prof_mask_;
@@ -910,13 +948,17 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     auto time_shape = [&](const Tuning &t, double out[4]) {
         tune_ = t;
         for (int k = 0; k < 5 && !rc; ++k) {
-            hipError_t e = hipMemcpyAsync(state[k], save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
+            hipError_t e = hipMemcpyAsync(state(k), save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
             if (e != hipSuccess) rc = fail(LF_ERR_HIP, "autotune: %s", hipGetErrorString(e));
         }
         if (rc) return;
         enqueue_substep();  // warm-up, untimed
+        advance_state();
         for (int k = 0; k < 4; ++k) prof_mask_ |= 1u << kinds[k];
-        for (uint32_t r = 0; r < reps; ++r) enqueue_substep();
+        for (uint32_t r = 0; r < reps; ++r) {
+            enqueue_substep();
+            advance_state();
+        }
         prof_mask_ = 0;
         if ((rc = check_launch("autotune"))) return;
         uint64_t cnt[LF_K_COUNT];
@@ -966,7 +1008,7 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     }
     prof_live_.swap(keep_live);
     for (int k = 0; k < 5; ++k)
-        (void)hipMemcpyAsync(state[k], save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
+        (void)hipMemcpyAsync(state(k), save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
     hipError_t e = hipStreamSynchronize(stream_);
     (void)hipFree(save);
     scenes_ = keep;
