@@ -652,11 +652,13 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     const float4 pi = Ps[i], vi = Vs[i];
     const float4 di = D[i];
     float3 fP = make_float3(0.f, 0.f, 0.f), fV = fP, fT = fP, fA = fP;
+    double eP[3] = {0.0, 0.0, 0.0};  // pressure sum of near-isolated pairs (FP64)
     if (i < st[c.ncells] + shift) {
         const float4 ni = c.has_tension ? N[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const float rho_i = di.x, p_i = di.y, cs_i = di.z, iri = di.w;
         const float iri2 = iri * iri;
         float Px = 0.f, Py = 0.f, Pz = 0.f;  // pressure sum
+        double Pdx = 0.0, Pdy = 0.0, Pdz = 0.0;  // its near-isolated pairs, in FP64
         float Qx = 0.f, Qy = 0.f, Qz = 0.f;  // artificial-viscosity part of the viscosity sum
         float Tx = 0.f, Ty = 0.f, Tz = 0.f;  // cohesion + curvature sum
         uint32_t b[9], e[9];
@@ -675,7 +677,9 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 float p;
                 float4 v, d, n;
             };
-            walk_rows<BS, kForceWalkWidth>(
+            // (two set bits per step in the near-isolated walk: its FP64
+            // terms would otherwise cost every scene's walk registers)
+            walk_rows<BS, F ? 2 : kForceWalkWidth>(
                 tab, b, e, M, c.use_masks, flat, nslots, i,
                 [&](uint32_t j) {
                     Cand cd;
@@ -699,6 +703,8 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     const float r = rr * inv_r;
                     const float irj = q.w;
                     float hr = fmaxf(c.h - r, 0.0f);
+                    bool frag = false;
+                    double hrd = 0.0;
                     if constexpr (F) {
                         if (fragile_i || irj > c.fragile_ir) {
                             // Near-isolated pair: h - r in FP64, r from the
@@ -706,15 +712,29 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                             // (relative error squared: ~1e-14).
                             const double d2 = dist2_d(pi, q), r0 = (double)r;
                             const double rd = __builtin_fma(__builtin_fma(-r0, r0, d2), (double)(0.5f * inv_r), r0);
-                            hr = (float)fmax(c.h_d - rd, 0.0);
+                            hrd = fmax(c.h_d - rd, 0.0);
+                            hr = (float)hrd;
+                            frag = true;
                         }
                     }
                     const float cm = c.mCgs * hr * hr * inv_r;  // m gradSpiky(r) / r
-                    const float mid = pj * iri2 + p_i * (irj * irj);
-                    const float cpr = cm * mid;
-                    Px += cpr * rx;
-                    Py += cpr * ry;
-                    Pz += cpr * rz;
+                    if (frag) {
+                        // and its pressure term in FP64: 1/rho^2 of a density
+                        // near FLT_MIN overflows FP32 (an infinite force).
+                        const double cmd = (double)c.mCgs * hrd * hrd * (double)inv_r;
+                        const double midd = (double)pj * ((double)iri * (double)iri) +
+                                            (double)p_i * ((double)irj * (double)irj);
+                        const double cprd = cmd * midd;
+                        Pdx += cprd * (double)rx;
+                        Pdy += cprd * (double)ry;
+                        Pdz += cprd * (double)rz;
+                    } else {
+                        const float mid = pj * iri2 + p_i * (irj * irj);
+                        const float cpr = cm * mid;
+                        Px += cpr * rx;
+                        Py += cpr * ry;
+                        Pz += cpr * rz;
+                    }
                     if constexpr (V) {
                         // Approaching pairs only (ux < 0); the others add exact zeros.
                         const float4 vj = cj.v, dj = cj.d;
@@ -756,6 +776,9 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
         else
             terms.template operator()<false>();
         fP = make_float3(Px, Py, Pz);
+        eP[0] = Pdx;
+        eP[1] = Pdy;
+        eP[2] = Pdz;
         fV = make_float3(c.nu * (Px + Qx), c.nu * (Py + Qy), c.nu * (Pz + Qz));
         // 2 rho0 / (rho_i + rho_i) applied once to the summed tension terms.
         const float ts = c.sigma * c.rho0 * iri;
@@ -772,9 +795,13 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
             }
         }
     }
+    // The viscosity force carries the pressure sum too (nu * (P + Pi)).
+    const double eV[3] = {(double)c.nu * eP[0], (double)c.nu * eP[1], (double)c.nu * eP[2]};
     if (Fdbg) {
-        Fdbg[4 * (size_t)i + 0] = make_float4(fP.x, fP.y, fP.z, 0.f);
-        Fdbg[4 * (size_t)i + 1] = make_float4(fV.x, fV.y, fV.z, 0.f);
+        Fdbg[4 * (size_t)i + 0] = make_float4((float)((double)fP.x + eP[0]), (float)((double)fP.y + eP[1]),
+                                              (float)((double)fP.z + eP[2]), 0.f);
+        Fdbg[4 * (size_t)i + 1] = make_float4((float)((double)fV.x + eV[0]), (float)((double)fV.y + eV[1]),
+                                              (float)((double)fV.z + eV[2]), 0.f);
         Fdbg[4 * (size_t)i + 2] = make_float4(fT.x, fT.y, fT.z, 0.f);
         Fdbg[4 * (size_t)i + 3] = make_float4(fA.x, fA.y, fA.z, 0.f);
     }
@@ -782,9 +809,9 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     // inputs: a' = F m + g (clamped to acc_limit); x += v dt + a dt^2/2 on the
     // double-float position; v += (a + a') dt/2 (clamped to vel_limit); then
     // the boundary projection and reflection.
-    const double Fx = (double)fP.x + (double)fT.x + (double)fA.x + (double)fV.x;
-    const double Fy = (double)fP.y + (double)fT.y + (double)fA.y + (double)fV.y;
-    const double Fz = (double)fP.z + (double)fT.z + (double)fA.z + (double)fV.z;
+    const double Fx = ((double)fP.x + eP[0]) + (double)fT.x + (double)fA.x + ((double)fV.x + eV[0]);
+    const double Fy = ((double)fP.y + eP[1]) + (double)fT.y + (double)fA.y + ((double)fV.y + eV[1]);
+    const double Fz = ((double)fP.z + eP[2]) + (double)fT.z + (double)fA.z + ((double)fV.z + eV[2]);
     double ax = Fx * c.m + c.g[0], ay = Fy * c.m + c.g[1], az = Fz * c.m + c.g[2];
     double s2 = ax * ax + ay * ay + az * az;
     if (s2 > c.alim2) {

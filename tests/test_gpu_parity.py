@@ -283,6 +283,36 @@ def test_isolated_pair_at_the_kernel_edge(gpu, oracle):
     assert g["density"][2] == np.float32(cfg.params.rest_density)  # alone: rest density
 
 
+def test_pair_one_ulp_inside_the_kernel_edge(gpu, oracle):
+    """Two particles as close inside h as FP32 positions near x = 0.025 allow
+    (1 - r/h ~ 1e-7; both more than h from the walls), one far away: each
+    density is ~1e-18, so p / rho^2 (~1e41) overflows FP32.  The pair's
+    pressure term is taken in FP64, so the accelerations stay finite and,
+    clamped to acc_limit, point where the oracle's do; densities, pressures
+    and pressure forces match it per particle."""
+    cfg = configs.dambreak_c1()
+    h = np.float32(cfg.params.h)
+    h2 = np.float32(h * h)
+    a = np.array([0.025, 0.25, 0.25], np.float32)
+    bx = np.float32(a[0] + h)
+    while not np.float32(np.float32(bx - a[0]) ** 2) <= h2:  # the largest x inside the FP32 predicate
+        bx = np.nextafter(bx, np.float32(0.0))
+    b = np.array([bx, a[1], a[2]], np.float32)
+    pos = np.stack([a, b, np.array([0.5, 0.3, 0.3], np.float32)]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    acc = np.zeros_like(pos)
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    o = oracle_substep(oracle, cfg, pos, vel, acc)
+    assert 0.0 < o["density"][0] < 1e-15, "the pair must be in the overflow regime"
+    assert np.all(np.isfinite(g["acc"])) and np.all(np.isfinite(g["vel"])), "non-finite state"
+    for k in ("density", "pressure", "f_pressure", "acc"):
+        gv, ov = np.asarray(g[k], np.float64), np.asarray(o[k], np.float64)
+        for i in range(2):
+            rel = np.linalg.norm(np.atleast_1d(gv[i] - ov[i])) / np.linalg.norm(np.atleast_1d(ov[i]))
+            assert rel <= 1e-5, f"{k} of particle {i}: relative error {rel:.3e}"
+    assert abs(float(np.linalg.norm(g["acc"][0])) - float(cfg.params.acc_limit)) < 1e-3
+
+
 def test_full_size_spray_c3(gpu, oracle_omp):
     """8,388,608 particles (256x115x256 block plus a 10% uniform spray layer
     above it: dense and sparse cells): bit-exact sort stages and TOL on one
