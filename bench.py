@@ -253,7 +253,7 @@ def cpu_baseline(cfg, target_s: float) -> dict:
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU only; --gpus {args.gpus} is not supported "
               f"(the single-scene path is not sharded across GPUs)", file=sys.stderr)
         sys.exit(2)
 

@@ -1,8 +1,8 @@
 // lf_oracle.cpp -- FP64 CPU restatement of one WCSPH substep.
 //
-// TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
-// cpu_baseline leg of bench.py as the checker / timed CPU baseline.  The
-// product never links it.
+// TEST INFRASTRUCTURE ONLY: used by tests/ and __graft_entry__.smoke() as the
+// reference the GPU results are compared against, and by the cpu_baseline
+// leg of bench.py as the timed CPU baseline.  The product never links it.
 //
 // Written from the formulas of the reference (citations are to
 // NCCA/libfluid), not from its source text.  State is kept in particle-id

@@ -3,8 +3,9 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (libfluid_amd, include/,
  * the fluidtest driver) links, loads or calls this library.  It is used by
- * tests/ as the checker, by __graft_entry__.smoke() as the checker, and by
- * bench.py's cpu_baseline leg as the timed CPU restatement.
+ * tests/ and __graft_entry__.smoke() as the reference the GPU results are
+ * compared against, and by bench.py's cpu_baseline leg as the timed CPU
+ * restatement.
  *
  * It restates, in double precision and single-threaded C++ (plus an OpenMP
  * build of the same source), the per-substep algorithm of
