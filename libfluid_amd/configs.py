@@ -172,4 +172,12 @@ def batched_c4(seed: int = 4, scenes: int = 256, lo: int = 2048, hi: int = 65536
     return BatchConfig(f"batched_c4_{scenes}", out)
 
 
+def spray_only(n: int = 262_144, seed: int = 7) -> Config:
+    """Spray alone: n points uniform in the middle of a 1.28 m box (grid
+    64^3), a few neighbours each, so near-isolated particles are everywhere."""
+    p = _plain_params((1.28, 1.28, 1.28), (64, 64, 64))
+    pos = uniform_points(n, (0.1, 0.1, 0.1), (1.18, 1.18, 1.18), seed)
+    return Config("spray_only", p, pos, np.zeros_like(pos), 1e-4, 150)
+
+
 CONFIGS = {"c1": dambreak_c1, "c2": column_c2, "c3": spray_c3, "double_dam": double_dam}
