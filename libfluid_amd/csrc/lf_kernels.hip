@@ -267,13 +267,13 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // skin radius.  walk_masks visits the set bits in increasing slot order, the
 // same order as a full scan; a row longer than 64 slots has no mask and is
 // scanned in full.  With use == false every row is scanned.  Each step takes
-// two set bits and issues both candidates' loads (ld) before using them
-// (fn, in slot order), so two gathers per lane are in flight at a time.
-// fn(cand, slot, ok) adds nothing when ok is false (a step with one set bit
-// left passes its first candidate again with ok = false); calling it
-// unconditionally keeps the second candidate's loads ahead of the first
-// candidate's arithmetic.
-template <class Ld, class Fn>
+// up to W set bits of the row and issues all their candidates' loads (ld)
+// before using them (fn, in slot order), so W gathers per lane are in flight
+// at a time.  fn(cand, slot, ok) adds nothing when ok is false (a step with
+// fewer set bits left passes its first candidate again with ok = false);
+// calling it unconditionally keeps the later candidates' loads ahead of the
+// first candidate's arithmetic.
+template <int W, class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
                                            uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
     uint64_t mk[9];
@@ -290,15 +290,22 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
         if (use && e[r] - b[r] <= 64u) {
             uint64_t m = mk[r];
             while (m) {
-                const uint32_t j1 = b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                uint32_t j[W];
+                bool ok[W];
+                j[0] = b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                ok[0] = true;
                 m &= m - 1ull;
-                const bool two = m != 0ull;
-                const uint32_t j2 = two ? b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u : j1;
-                m &= m - 1ull;  // stays 0 when it was 0
-                const auto a1 = ld(j1);
-                const auto a2 = ld(j2);
-                fn(a1, j1, true);
-                fn(a2, j2, two);
+#pragma unroll
+                for (int k = 1; k < W; ++k) {
+                    ok[k] = m != 0ull;
+                    j[k] = ok[k] ? b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u : j[0];
+                    m &= m - 1ull;  // stays 0 when it was 0
+                }
+                decltype(ld(0u)) a[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) a[k] = ld(j[k]);
+#pragma unroll
+                for (int k = 0; k < W; ++k) fn(a[k], j[k], ok[k]);
             }
         } else {
             for (uint32_t j = b[r]; j < e[r]; ++j)
@@ -326,6 +333,7 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 constexpr int kFlatEntries = 16;
 constexpr int kForceWalkWidth = 4;  // set bits per step of the flattened force walk
 constexpr int kXsphWalkWidth = 4;   // and of the flattened XSPH walk
+constexpr int kXsphRowWidth = 4;    // set bits per step of the row-by-row XSPH walk
 template <int BS, int W, class Ld, class Fn>
 __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, uint32_t nslots, uint32_t i, bool need_long, Ld &&ld,
@@ -406,12 +414,12 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
 // Walk over the neighbour slots of lane i: with flat set, the flattened walk
 // when masks are in use and every lane's rows fit it, else walk_masks.  The
 // choice is wave-uniform and does not change the result.
-template <int BS, int W, class Ld, class Fn>
+template <int BS, int W, int WR, class Ld, class Fn>
 __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, bool use, int flat, uint32_t nslots, uint32_t i,
                                           Ld &&ld, Fn &&fn) {
     if (use && flat && walk_flat<BS, W>(tab, b, e, mask, nslots, i, flat == 2, ld, fn)) return;
-    walk_masks(b, e, mask, use, nslots, i, ld, fn);
+    walk_masks<WR>(b, e, mask, use, nslots, i, ld, fn);
 }
 
 // 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
@@ -607,7 +615,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
     float sx = 0.0f, sy = 0.0f, sz = 0.0f;
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
-        walk_rows<BS, 2>(
+        walk_rows<BS, 2, 2>(
             tab, b, e, M, c.use_masks, flat, nslots, i, [&](uint32_t j) { return at32(Q, j); },
             [&](const float4 &q, uint32_t, bool ok) { normal_term<F>(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     };
@@ -679,7 +687,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
             };
             // (two set bits per step in the near-isolated walk: its FP64
             // terms would otherwise cost every scene's walk registers)
-            walk_rows<BS, F ? 2 : kForceWalkWidth>(
+            walk_rows<BS, F ? 2 : kForceWalkWidth, 2>(
                 tab, b, e, M, c.use_masks, flat, nslots, i,
                 [&](uint32_t j) {
                     Cand cd;
@@ -907,7 +915,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     };
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
-        walk_rows<BS, kXsphWalkWidth>(
+        walk_rows<BS, F ? 2 : kXsphWalkWidth, F ? 2 : kXsphRowWidth>(
             tab, b, e, MS, masks_ok, flat, nslots, i,
             [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
             [&](const Cand &cj, uint32_t j, bool ok) {
