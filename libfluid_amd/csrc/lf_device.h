@@ -14,6 +14,7 @@ constexpr uint32_t kMaxParticles = 1u << 28;  // pool slots: 16-byte records add
 constexpr uint32_t kRankFixMax = 4096;        // largest bucket the in-cell id ranking handles
 constexpr int kScanPerThread = 4;             // cell-count scan: entries per thread (one 16-byte load)
 constexpr int kScanTile = kBlock * kScanPerThread;
+constexpr uint32_t kScanDirectTiles = 2048;    // above this many tiles the tile prefixes get their own pass
 
 // Neighbour masks of one radius for 9 stencil rows x nslots particle slots,
 // row-major (entry r * nslots + i).  Bit k stands for slot b_r + k of row r.

@@ -17,12 +17,14 @@ struct Launch {
 
 // Device scene table of a pool: cs[s] are the constants of scene s, chunk[c]
 // the scene owning slots [512 c, 512 c + 512) (kNoScene if none); nslots is
-// the pool's slot count (a multiple of 512).  Stepping launches cover all
-// nslots slots.
+// the pool's slot count (a multiple of 512), the stride of its per-slot
+// planes.  Stepping launches cover slots [0, nlaunch) (a multiple of 512, at
+// most nslots), which hold every active scene.
 struct PoolView {
     const Consts *cs;
     const uint32_t *chunk;
     uint32_t nslots;
+    uint32_t nlaunch;
 };
 
 uint32_t scan_tiles(uint32_t m);
@@ -31,6 +33,8 @@ bool valid_block_size(int bs);
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts, uint32_t *flags, uint32_t *moves);
+// Exclusive scan of counts[0, m) into start[0, m] (start[m] = total), which
+// also clears the counts; bsum holds scan_tiles(m) + 1 words.
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);

@@ -126,18 +126,39 @@ __global__ __launch_bounds__(kBlock) void k_scan_sum(const uint32_t *__restrict_
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// Phase 2: each tile adds up the sums of the tiles before it (at most a few
-// thousand, cached), scans its own buckets, writes their starts and clears
-// their counts for the next substep; the last tile writes the end sentinel
-// start[m].
+// Phase 1b, for more than kScanDirectTiles tiles: one workgroup turns the
+// tile sums into exclusive prefixes in place, kScanTile of them per step.
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(uint32_t *__restrict__ bsum, uint32_t nb) {
+    __shared__ uint32_t sh[kBlock / 64];
+    uint32_t carry = 0;
+    for (uint32_t t0 = 0; t0 < nb; t0 += kScanTile) {
+        const uint32_t base = t0 + threadIdx.x * kScanPerThread;
+        const uint4 v = scan_load(bsum, base, nb);
+        uint32_t tot;
+        const uint32_t run = carry + block_excl_scan(v.x + v.y + v.z + v.w, sh, tot);
+        const uint32_t sv[4] = {run, run + v.x, run + v.x + v.y, run + v.x + v.y + v.z};
+        for (uint32_t k = 0; k < 4u; ++k)
+            if (base + k < nb) bsum[base + k] = sv[k];
+        carry += tot;
+    }
+}
+
+// Phase 2: each tile takes the sum of the tiles before it (prefixed: phase
+// 1b's prefix; else it adds up their sums, at most kScanDirectTiles, cached),
+// scans its own buckets, writes their starts and clears their counts for the
+// next substep; the last tile writes the end sentinel start[m].
 __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ counts, uint32_t m,
                                                       const uint32_t *__restrict__ bsum, uint32_t nb,
-                                                      uint32_t *__restrict__ start) {
+                                                      int prefixed, uint32_t *__restrict__ start) {
     __shared__ uint32_t sh[kBlock / 64];
-    uint32_t before = 0;
-    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) before += bsum[t];
     uint32_t prefix;
-    (void)block_excl_scan(before, sh, prefix);
+    if (prefixed) {
+        prefix = bsum[blockIdx.x];
+    } else {
+        uint32_t before = 0;
+        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) before += bsum[t];
+        (void)block_excl_scan(before, sh, prefix);
+    }
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
     const uint4 v = scan_load(counts, base, m);
     uint32_t tot;
@@ -1204,27 +1225,29 @@ bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts, uint32_t *flags, uint32_t *moves) {
-    LF_GO(L, k_hash, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves);
+    LF_GO(L, k_hash, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
 
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start) {
     const uint32_t nb = scan_tiles(m);
+    const bool prefixed = nb > kScanDirectTiles;
     LF_GO(L, k_scan_sum, nb, kBlock, L.a, nullptr, (const uint32_t *)counts, m, bsum);
-    LF_GO(L, k_scan_down, nb, kBlock, nullptr, L.b, counts, m, (const uint32_t *)bsum, nb, start);
+    if (prefixed) LF_GO(L, k_scan_tiles, 1, kBlock, nullptr, nullptr, bsum, nb);
+    LF_GO(L, k_scan_down, nb, kBlock, nullptr, L.b, counts, m, (const uint32_t *)bsum, nb, prefixed ? 1 : 0, start);
 }
 
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id) {
-    LF_GO(L, k_scatter, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, start, slot_src,
+    LF_GO(L, k_scatter, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, start, slot_src,
           slot_id);
 }
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
                    float4 *Ps, float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
-    LF_GO(L, k_gather, pv.nslots / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
+    LF_GO(L, k_gather, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
           A, Ps, Vs, order, soa, pv.nslots, moves, err);
 }
 
@@ -1232,7 +1255,7 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
                     uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_density<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
+        LF_GO(L, k_density<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
               MS, flags, moves);
     });
 }
@@ -1240,7 +1263,7 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
 void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool all, const float4 *Q,
                     const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_normals<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat,
+        LF_GO(L, k_normals<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat,
               all ? 1 : 0, flags, N);
     });
 }
@@ -1250,7 +1273,7 @@ void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const 
                    const float4 *D, const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout,
                    float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_forces<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, A, Lw, order,
+        LF_GO(L, k_forces<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, A, Lw, order,
               D, N, start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
     });
 }
@@ -1259,7 +1282,7 @@ void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const fl
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_xsph<BS>, pv.nslots / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
+        LF_GO(L, k_xsph<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
               flat, flags, Vout);
     });
 }

@@ -144,11 +144,17 @@ class Pool {
     template <class Fn>
     void timed(int kind, Fn &&launch);
     void drop_graph();
-    PoolView view() const { return PoolView{d_cs_, d_chunk_, nslots_}; }
+    PoolView view() const { return PoolView{d_cs_, d_chunk_, nslots_, span_slots_}; }
+    void set_spans();
 
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     uint32_t nslots_ = 0, nbuckets_ = 0, ntiles_ = 0;
+    // Slots and buckets the stepping launches cover, from the tables last
+    // pushed: the end of the highest active scene's chunks and of its bucket
+    // range, each rounded up to a sixteenth of the pool (so that a captured
+    // substep stays valid while scenes come and go below that mark).
+    uint32_t span_slots_ = 0, span_buckets_ = 0;
     int debug_ = 0;
     Tuning tune_{};
     bool use_graphs_ = true;
@@ -187,6 +193,7 @@ class Pool {
     int parity_ = 0;              // which pairing is current (toggled by advance_state)
     bool graph_normals_ = false;  // the captured substeps have the normals pass
     std::vector<std::pair<uint32_t, uint32_t>> graph_paused_;  // and carry these paused scenes' slots
+    std::pair<uint32_t, uint32_t> graph_span_{0, 0};         // and cover these spans
 
     Exporter *exporter_ = nullptr;
 

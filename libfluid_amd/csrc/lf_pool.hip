@@ -152,6 +152,8 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if ((uint64_t)bucket_cap + 1 >= (1ull << 32)) return fail(LF_ERR_INVALID, "pool: too many buckets");
     nslots_ = (uint32_t)slots;
     nbuckets_ = bucket_cap;
+    span_slots_ = nslots_;
+    span_buckets_ = nbuckets_;
     ntiles_ = scan_tiles(nbuckets_);
     scenes_.assign(scene_cap, SceneRec{});
     h_cs_.assign(scene_cap, Consts{});
@@ -235,6 +237,7 @@ int Pool::push_tables() {
         if (s.active)
             for (uint32_t k = 0; k < s.nchunks; ++k) h_chunk_[s.first / kChunk + k] = id;
     }
+    set_spans();
     const int k = table_slot_;
     table_slot_ = (table_slot_ + 1) % kTableRing;
     if (tables_ev_pending_[k]) LF_HIP(hipEventSynchronize(tables_ev_[k]));
@@ -247,6 +250,20 @@ int Pool::push_tables() {
     tables_ev_pending_[k] = true;
     tables_dirty_ = false;
     return LF_OK;
+}
+
+void Pool::set_spans() {
+    const uint32_t gs = ((nslots_ / 16 + kChunk - 1) / kChunk) * kChunk, gb = (nbuckets_ + 15) / 16;
+    uint64_t es = 0, eb = 0;
+    for (const SceneRec &s : scenes_)
+        if (s.live && s.active) {
+            es = std::max<uint64_t>(es, s.first + (uint64_t)s.nchunks * kChunk);
+            eb = std::max<uint64_t>(eb, s.cell_base + (uint64_t)s.ncells + 1);
+        }
+    es = std::max<uint64_t>(es, 1);
+    eb = std::max<uint64_t>(eb, 1);
+    span_slots_ = (uint32_t)std::min<uint64_t>(nslots_, (es + gs - 1) / gs * gs);
+    span_buckets_ = (uint32_t)std::min<uint64_t>(nbuckets_, (eb + gb - 1) / gb * gb);
 }
 
 const SceneRec *Pool::scene(uint32_t id) const {
@@ -474,7 +491,7 @@ bool Pool::any_separate_normals() const {
 void Pool::enqueue_sort() {
     const PoolView pv = view();
     timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_); });
-    timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, nbuckets_, bsum_, start_); });
+    timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, span_buckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
         launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, P1_, V1_, ranks_, soa_, moves_, err_);
@@ -549,8 +566,9 @@ int Pool::run_substeps(uint32_t substeps) {
         }
         return check_launch("step");
     }
+    const std::pair<uint32_t, uint32_t> span{span_slots_, span_buckets_};
     if ((graph_exec_[0] || graph_exec_[1]) &&
-        (graph_normals_ != any_separate_normals() || graph_paused_ != paused_ranges()))
+        (graph_normals_ != any_separate_normals() || graph_paused_ != paused_ranges() || graph_span_ != span))
         drop_graph();
     for (uint32_t k = 0; k < substeps; ++k) {
         hipGraphExec_t &ge = graph_exec_[parity_];
@@ -572,6 +590,7 @@ int Pool::run_substeps(uint32_t substeps) {
             graph_[parity_] = g;
             graph_normals_ = any_separate_normals();
             graph_paused_ = paused_ranges();
+            graph_span_ = span;
         }
         LF_HIP(hipGraphLaunch(ge, stream_));
         advance_state();
