@@ -37,6 +37,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -281,7 +282,13 @@ def main():
     sc.sync()
     log("[bench] warmup done")
 
-    # Timed region: K steps; the force kernel is bracketed by HIP events.
+    # The state the timed region starts from, replayed below with graphs.
+    fd, ck_path = tempfile.mkstemp(prefix="lf_bench_", suffix=".lfck")
+    os.close(fd)
+    sc.checkpoint_save(ck_path)
+
+    # Timed region: K steps; the force kernel is bracketed by HIP events
+    # (the profiled substeps are launched one kernel at a time).
     sc.profile(["forces"])
     sc.sync()
     t0 = time.perf_counter()
@@ -309,16 +316,21 @@ def main():
     sc.profile([])
     kernels_ms = {k: (ms / c if c else None) for k, (c, ms) in per.items()}
 
-    # Same scene, no profiling: each substep replays the captured hipGraph.
-    sc.step(cfg.dt, 1)
+    sc.close()
+
+    # The same K steps from the same state, unprofiled: each substep replays
+    # the captured hipGraph (the first one captures it, inside the timing).
+    sc = lf.Scene.checkpoint_load(ck_path)
+    os.unlink(ck_path)
+    if args.scan:
+        sc.set_debug(forces=False, scan=True)
     sc.sync()
-    ng = min(200, args.steps)
     t0 = time.perf_counter()
-    for _ in range(ng):
+    for _ in range(args.steps):
         sc.step(cfg.dt, 1)
     sc.sync()
-    graph_ms = 1e3 * (time.perf_counter() - t0) / ng
-    log(f"[bench] graph replay: {graph_ms:.4f} ms/step")
+    graph_ms = 1e3 * (time.perf_counter() - t0) / args.steps
+    log(f"[bench] graph replay of the timed steps: {graph_ms:.4f} ms/step")
 
     # Per-step latency (each step synchronised).
     lat = []
