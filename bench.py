@@ -30,6 +30,11 @@ HBM budget (--serve-budget-gb).  The same jobs all submitted at once
 stream is then offered 0.9 of it.  Reported: aggregate particle-steps/s,
 p50/p99 per-step and per-job latency, mean occupancy, and the rate of the
 same jobs stepped alone on a sample of them.
+
+io: on the column scene, the lattice generated on the device against host
+generation plus upload; an exact checkpoint saved and loaded; a .geo export
+written synchronously, and issued asynchronously ahead of as many substeps
+as the synchronous write took, against the same substeps without it.
 """
 from __future__ import annotations
 
@@ -69,6 +74,7 @@ def parse():
                     help="after the run, launch the plain copy kernel once over MIB MiB (calibrates byte counters)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scene measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the served-stream measurement")
+    ap.add_argument("--no-io", action="store_true", help="skip the setup, checkpoint and export measurement")
     ap.add_argument("--serve-budget-gb", type=float, default=1.0, help="device memory budget of the server")
     ap.add_argument("--serve-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=200)
@@ -258,6 +264,96 @@ def cpu_baseline(cfg, target_s: float) -> dict:
     }
 
 
+def io_leg(args) -> dict:
+    """The steps either side of the substep on the column scene: device-side
+    lattice setup against host generation plus upload, checkpoint save and
+    load, and a .geo export, synchronous and overlapped with stepping."""
+    import shutil
+
+    import libfluid_amd as lf
+    from libfluid_amd import _lib as L
+    from libfluid_amd import configs
+
+    cfg = configs.CONFIGS[args.config]()
+    if cfg.name != "column_c2":
+        return None
+    n, p = cfg.n, cfg.params
+    lattice = dict(nx=128, ny=64, nz=128, dx=configs.DX, origin=(0.0, 0.0, 0.0), jitter=0.05 * configs.DX, seed=2)
+    tmp = tempfile.mkdtemp(prefix="lf_bench_io_")
+    try:
+        # Setup: the lattice generated on the device, or on the host and uploaded.
+        dev = lf.Scene(p, n)
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.setup_jittered_lattice(**lattice)
+        dev.sync()
+        t_dev = time.perf_counter() - t0
+        host = lf.Scene(p, n)
+        host.sync()
+        t0 = time.perf_counter()
+        pos = lf.jittered_lattice(**lattice)
+        host.upload(pos, np.zeros_like(pos))
+        host.sync()
+        t_host = time.perf_counter() - t0
+        same = bool(np.array_equal(dev.download()["pos"], host.download()["pos"]))
+        host.close()
+        log(f"[bench] io: setup {n} particles on the device {1e3 * t_dev:.2f} ms, on the host + upload "
+            f"{1e3 * t_host:.2f} ms, bitwise equal: {same}")
+        sc = dev
+        sc.run(cfg.dt, 20)
+        sc.sync()
+        # Exact checkpoint: save (device to file) and load (file to a new scene).
+        ck = os.path.join(tmp, "state.lfck")
+        t0 = time.perf_counter()
+        sc.checkpoint_save(ck)
+        t_save = time.perf_counter() - t0
+        ck_bytes = os.path.getsize(ck)
+        t0 = time.perf_counter()
+        back = lf.Scene.checkpoint_load(ck)
+        back.sync()
+        t_load = time.perf_counter() - t0
+        back.close()
+        log(f"[bench] io: checkpoint {ck_bytes} B, save {1e3 * t_save:.1f} ms, load {1e3 * t_load:.1f} ms")
+        # .geo export: the synchronous path (download + formatted write) ...
+        t0 = time.perf_counter()
+        st = sc.download(L.LF_ORDER_CELL)
+        lf.write_geo(os.path.join(tmp, "sync"), 1, st["pos"], st["colour"], st["vel"], float(p.particle_size))
+        t_sync = time.perf_counter() - t0
+        geo_bytes = os.path.getsize(os.path.join(tmp, "sync.0001.geo"))
+        # ... and the asynchronous one, issued before as many substeps as the
+        # synchronous export takes, against the same substeps without it.
+        t0 = time.perf_counter()
+        sc.run(cfg.dt, 50)
+        sc.sync()
+        step_s = (time.perf_counter() - t0) / 50
+        k = int(min(20000, max(100, t_sync / step_s)))
+        t0 = time.perf_counter()
+        sc.run(cfg.dt, k)
+        sc.sync()
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sc.export_geo_async(os.path.join(tmp, "async"), 1)
+        sc.run(cfg.dt, k)
+        sc.sync()
+        t_steps = time.perf_counter() - t0
+        sc.export_wait()
+        t_overlap = time.perf_counter() - t0
+        sc.close()
+        log(f"[bench] io: .geo {geo_bytes} B, synchronous {1e3 * t_sync:.0f} ms; {k} substeps {1e3 * t_plain:.0f} ms "
+            f"alone, {1e3 * t_steps:.0f} ms with an asynchronous export in flight (export done at "
+            f"{1e3 * t_overlap:.0f} ms)")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    return {
+        "workload": f"{cfg.name}: {n} particles",
+        "setup_device_ms": 1e3 * t_dev, "setup_host_upload_ms": 1e3 * t_host, "setup_bitwise_equal": same,
+        "checkpoint_bytes": ck_bytes, "checkpoint_save_ms": 1e3 * t_save, "checkpoint_load_ms": 1e3 * t_load,
+        "geo_bytes": geo_bytes, "geo_sync_ms": 1e3 * t_sync,
+        "overlap_substeps": k, "substeps_alone_ms": 1e3 * t_plain,
+        "substeps_with_async_export_ms": 1e3 * t_steps, "async_export_done_ms": 1e3 * t_overlap,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -392,6 +488,7 @@ def main():
     }
     out["batched"] = None if args.no_batch else batched(args)
     out["served"] = None if args.no_serve else served(args)
+    out["io"] = None if args.no_io else io_leg(args)
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
