@@ -121,6 +121,41 @@ def test_admission_removal_compaction_and_pause(gpu):
         assert_same(b.download(ids[4]), solo(gpu, cfgs[4], 8), "resumed scene")
 
 
+def test_spare_capacity_spans_and_long_scans(gpu):
+    """A pool with far more room than its scenes.  A filler scene (100,000
+    slots, a 150^3 grid) is admitted first and scene A after it, so A lies
+    high in both slot and bucket space once the filler leaves; B then takes
+    the bottom (first fit).  Stepping launches cover only the active span,
+    whose bucket part here needs more than 2,048 scan tiles (the tile-prefix
+    pass); pausing A shrinks the span and resuming it grows it again (each a
+    new capture).  Every scene matches its solo run bit for bit."""
+    a, bcfg = mixed_scenes(2, seed=12)
+    filler = configs.Config("filler", configs._plain_params((3.0, 3.0, 3.0), (150, 150, 150)),
+                            np.zeros((100_000, 3), np.float32), np.zeros((100_000, 3), np.float32), 1e-4, 1)
+    assert filler.params.ncells + 1 > 2048 * 1024
+    with batch_for(gpu, [filler, a, bcfg], extra_slots=64 * 512, extra_cells=1_000_000) as b:
+        fid = b.add(filler.params, filler.n, filler.dt)
+        ida = b.add(a.params, a.n, a.dt)
+        b.remove(fid)
+        idb = b.add(bcfg.params, bcfg.n, bcfg.dt)
+        b.upload(ida, a.pos, a.vel)
+        b.upload(idb, bcfg.pos, bcfg.vel)
+        b.step(4)
+        b.sync()
+        assert_same(b.download(ida), solo(gpu, a, 4), "high scene")
+        assert_same(b.download(idb), solo(gpu, bcfg, 4), "low scene")
+        b.set_active(ida, False)
+        b.step(2)
+        b.sync()
+        assert_same(b.download(idb), solo(gpu, bcfg, 6), "low scene, high one paused")
+        assert_same(b.download(ida), solo(gpu, a, 4), "paused high scene")
+        b.set_active(ida, True)
+        b.step(3)
+        b.sync()
+        assert_same(b.download(ida), solo(gpu, a, 7), "high scene resumed")
+        assert_same(b.download(idb), solo(gpu, bcfg, 9), "low scene")
+
+
 def test_batch_full_reports_full(gpu):
     cfgs = mixed_scenes(2, seed=8)
     with batch_for(gpu, cfgs) as b:
