@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--serve-budget-gb", type=float, default=1.0, help="device memory budget of the server")
     ap.add_argument("--serve-jobs", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batched run only, without stepping the same scenes one after another")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--scan", action="store_true",
                     help="test every stencil slot in every pass instead of walking neighbour masks")
@@ -113,6 +115,21 @@ def batched(args) -> dict:
     b.close()
     log(f"[bench] batch: {1e3 * t_batch / K:.3f} ms/step, {bc.n * K / t_batch:.4e} particle-steps/s, tuning {tuning}")
 
+    value = bc.n * K / t_batch
+    out = {
+        "workload": f"{len(bc.scenes)} scenes, {bc.n} particles, {bc.cells} cells; counts log-uniform in "
+                    f"[2048, 65536], own box and gravity direction each, dt=1e-4",
+        "scenes": len(bc.scenes), "particles": bc.n, "steps": K,
+        "value": value, "unit": "particle-steps/s",
+        "scene_steps_per_s": len(bc.scenes) * K / t_batch,
+        "ms_per_step": 1e3 * t_batch / K,
+        "sequential_value": None, "sequential_ms_per_step": None,
+        "speedup_vs_sequential": None, "bitwise_equal_to_sequential": None,
+        "tuning": list(tuning),
+        "autotune_ms": np.round(tune_ms, 4).tolist(),
+    }
+    if args.no_sequential:
+        return out
     # Sequential: the same scenes, each alone, W + 1 + K substeps.
     t_seq = 0.0
     same = True
@@ -129,21 +146,10 @@ def batched(args) -> dict:
         same &= all(np.array_equal(d[k], final[j][k]) for k in ("pos", "vel", "acc"))
         sc.close()
     log(f"[bench] sequential: {1e3 * t_seq / K:.3f} ms per step of all scenes, bitwise equal: {same}")
-    value = bc.n * K / t_batch
     seq = bc.n * K / t_seq
-    return {
-        "workload": f"{len(bc.scenes)} scenes, {bc.n} particles, {bc.cells} cells; counts log-uniform in "
-                    f"[2048, 65536], own box and gravity direction each, dt=1e-4",
-        "scenes": len(bc.scenes), "particles": bc.n, "steps": K,
-        "value": value, "unit": "particle-steps/s",
-        "scene_steps_per_s": len(bc.scenes) * K / t_batch,
-        "ms_per_step": 1e3 * t_batch / K,
-        "sequential_value": seq, "sequential_ms_per_step": 1e3 * t_seq / K,
-        "speedup_vs_sequential": value / seq,
-        "bitwise_equal_to_sequential": bool(same),
-        "tuning": list(tuning),
-        "autotune_ms": np.round(tune_ms, 4).tolist(),
-    }
+    out.update(sequential_value=seq, sequential_ms_per_step=1e3 * t_seq / K, speedup_vs_sequential=value / seq,
+               bitwise_equal_to_sequential=bool(same))
+    return out
 
 
 def served(args) -> dict:
