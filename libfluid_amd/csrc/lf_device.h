@@ -124,11 +124,13 @@ __device__ __forceinline__ uint32_t *move_slot(uint32_t *moves, uint32_t scene, 
     return moves + ((size_t)scene * kMoveSlots + k) * kMoveStride;
 }
 constexpr double kSkinMin = 0.01, kSkinMax = 0.1;
-__device__ __forceinline__ uint32_t skin_quantum(const Consts &c, float4 v, float4 a) {
-    const float dt = (float)c.dt, hdt2 = 0.5f * (float)c.dtdt;
-    const float mx = __builtin_fmaf(a.x, hdt2, v.x * dt), my = __builtin_fmaf(a.y, hdt2, v.y * dt),
-                mz = __builtin_fmaf(a.z, hdt2, v.z * dt);
-    const float q = ceilf(sqrtf(__builtin_fmaf(mz, mz, __builtin_fmaf(my, my, mx * mx))) * c.skin_q) + 1.0f;
+// A particle's move in the substep, v dt + a dt^2 / 2, is bounded without
+// reading a by |v| dt + acc_limit dt^2 / 2: the integrator clamps |a| to
+// acc_limit (a larger uploaded acceleration is caught by the far-move check).
+__device__ __forceinline__ uint32_t skin_quantum(const Consts &c, float4 v) {
+    const float dt = (float)c.dt, ahdt2 = (float)(0.5 * c.dtdt * c.alim);
+    const float sv = sqrtf(__builtin_fmaf(v.z, v.z, __builtin_fmaf(v.y, v.y, v.x * v.x)));
+    const float q = ceilf(__builtin_fmaf(sv, dt, ahdt2) * c.skin_q) + 1.0f;
     return (uint32_t)fminf(q, (float)kSkinQMax);  // NaN -> kSkinQMax
 }
 // Skin fraction s (exact in FP32) for the scene's quantised largest move.

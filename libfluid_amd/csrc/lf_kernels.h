@@ -42,8 +42,8 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 // slot of the particle in slot f (acceleration and low position words stay
 // in the state arrays, read there by the force pass).
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
-                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   float4 *Ps, float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err);
+                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
+                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err);
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
                     uint32_t *flags, const uint32_t *moves);

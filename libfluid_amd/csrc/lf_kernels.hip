@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
                                                    const uint32_t *__restrict__ slot_id,
                                                    const uint32_t *__restrict__ keys,
                                                    const uint32_t *__restrict__ start, const float4 *__restrict__ P,
-                                                   const float4 *__restrict__ V, const float4 *__restrict__ A,
+                                                   const float4 *__restrict__ V,
                                                    float4 *__restrict__ Ps, float4 *__restrict__ Vs,
                                                    uint32_t *__restrict__ order, float *__restrict__ soa,
                                                    uint32_t nslots, uint32_t *__restrict__ moves,
@@ -237,14 +237,14 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
             f = s;  // bucket too large to rank: keep the provisional slot, report it
             if (s == lo) atomicOr(err, 1u);
         }
-        const float4 p = P[src], v = V[src], a = A[src];
+        const float4 p = P[src], v = V[src];
         Ps[f] = p;
         soa[f] = p.x;
         soa[nslots + f] = p.y;
         soa[2 * (size_t)nslots + f] = p.z;
         Vs[f] = v;
         order[f] = src;
-        q = skin_quantum(*cp, v, a);
+        q = skin_quantum(*cp, v);
     }
     // The scene's largest move of the substep (XSPH skin): reduced over the
     // workgroup, then one atomicMax into the slot of the workgroup's index.
@@ -1245,10 +1245,10 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 }
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
-                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, const float4 *A,
-                   float4 *Ps, float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
+                   const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
+                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
     LF_GO(L, k_gather, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          A, Ps, Vs, order, soa, pv.nslots, moves, err);
+          Ps, Vs, order, soa, pv.nslots, moves, err);
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,

@@ -494,7 +494,7 @@ void Pool::enqueue_sort() {
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, span_buckets_, bsum_, start_); });
     timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
     timed(LF_K_GATHER, [&](const Launch &L) {
-        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, A0_, P1_, V1_, ranks_, soa_, moves_, err_);
+        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, P1_, V1_, ranks_, soa_, moves_, err_);
     });
 }
 
