@@ -17,7 +17,8 @@
 //   L = low-order position words: the position is the double-float P.xyz + L.xyz
 //   D = (density, pressure, sound speed, 1/density)   N = (normal, 0)
 // Ids are scene-local.  The sorted arrays are in (cell key, particle id)
-// order; `start` holds the first slot of every bucket plus an end sentinel.
+// order; `start` holds the first slot of every bucket up to the pool's
+// active span plus an end sentinel.
 //
 // The density pass tests every candidate of the clipped 27-cell stencil (nine
 // contiguous x-row slot ranges) and records, per row, 32/64-bit masks of the
