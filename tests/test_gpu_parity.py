@@ -147,6 +147,39 @@ def test_replay_double_dam_preset(gpu, oracle_omp):
         compare_substep(g, o, float(np.float32(cfg.params.particle_size)), f"double dam from step {step}")
 
 
+def test_replay_out_of_grid_particles(gpu, oracle_omp):
+    """Particles beyond the grid on every side (NULL_HASH in the reference,
+    hash.cu:20-26): their key reads NULL_HASH (0xFFFFFFFF), they sort after
+    every cell and are nobody's neighbour; their density, normal and
+    forces are zero, and they are still integrated (gravity, then the wall
+    projection), fluidsystem.cu:369-386.  With the dam-break coefficients
+    on, so that every force term runs next to them."""
+    cfg = configs.scaled_c1(12, 10, 9)
+    cfg.params.viscosity, cfg.params.surface_tension, cfg.params.adhesion = 0.2, 1.0, 0.1
+    rng = np.random.Generator(np.random.PCG64(31))
+    box = np.array([1.0, 0.5, 0.5], np.float32)
+    out = []
+    for axis in range(3):
+        for side in (-1.0, 1.0):
+            q = rng.uniform(0.05, 0.3, (8, 3)).astype(np.float32)
+            q[:, axis] = np.float32(-0.004) if side < 0 else box[axis] + np.float32(0.004)
+            out.append(q)
+    far = np.concatenate(out)
+    pos = np.concatenate([cfg.pos, far]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    vel[-far.shape[0]:] = rng.normal(0.0, 0.5, far.shape).astype(np.float32)
+    acc = np.zeros_like(pos)
+    cfg = configs.Config(cfg.name + "+outside", cfg.params, pos, vel, cfg.dt, 1)
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
+    outside = np.arange(pos.shape[0]) >= pos.shape[0] - far.shape[0]
+    assert (o["keys"][outside] == 0xFFFFFFFF).all() and (o["keys"][~outside] < int(cfg.params.ncells)).all()
+    assert set(g["order"][-far.shape[0]:].tolist()) == set(np.flatnonzero(outside).tolist())
+    for k in ("density", "f_pressure", "f_viscosity", "f_tension", "f_adhesion"):
+        assert not np.any(g[k][outside]), f"{k} of an out-of-grid particle is not zero"
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "out-of-grid particles")
+
+
 def test_replay_dense_rows(gpu, oracle_omp):
     """A block at h/4 spacing: stencil rows of more than 64 slots (scanned in
     full) inside it, mask-walked rows of at most 64 slots at its faces."""
