@@ -180,6 +180,29 @@ def test_replay_out_of_grid_particles(gpu, oracle_omp):
     compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "out-of-grid particles")
 
 
+def test_replay_coincident_particles(gpu, oracle_omp):
+    """Exact collisions: copies of interior particles at the same position
+    (a pair, a triple and a quadruple), with different velocities.  Density
+    counts a coincident partner at full weight (j != i, r^2 <= h^2,
+    fluidphysics.cu:8-84); the normal and force sums skip it (r^2 > eps,
+    fluidphysics.cu:154-352); XSPH averages their velocities
+    (fluidphysics.cu:355-409).  The sort keeps them in id order."""
+    cfg = configs.scaled_c1(12, 10, 9)
+    cfg.params.viscosity, cfg.params.surface_tension, cfg.params.adhesion = 0.2, 1.0, 0.1
+    rng = np.random.Generator(np.random.PCG64(41))
+    src = [500, 501, 501, 620, 620, 620]
+    pos = np.concatenate([cfg.pos, cfg.pos[src]]).astype(np.float32)
+    vel = np.zeros_like(pos)
+    vel[cfg.n:] = rng.normal(0.0, 0.3, (len(src), 3)).astype(np.float32)
+    acc = np.zeros_like(pos)
+    cfg = configs.Config(cfg.name + "+coincident", cfg.params, pos, vel, cfg.dt, 1)
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
+    _, counts = np.unique(pos, axis=0, return_counts=True)
+    assert sorted(counts[counts > 1].tolist()) == [2, 3, 4]
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "coincident particles")
+
+
 def test_replay_dense_rows(gpu, oracle_omp):
     """A block at h/4 spacing: stencil rows of more than 64 slots (scanned in
     full) inside it, mask-walked rows of at most 64 slots at its faces."""
