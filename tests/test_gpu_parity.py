@@ -203,6 +203,48 @@ def test_replay_coincident_particles(gpu, oracle_omp):
     compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "coincident particles")
 
 
+def exact_lattice_scene():
+    """A binary-exact scene: h = cell = 1/32, spacing 1/64 = h/2, zero
+    jitter.  Every other lattice plane lies exactly on a cell face
+    (x * inv_cell is an exact integer there), axis neighbours two spacings
+    apart sit at exactly r = h (r^2 == h^2 in FP32 and FP64), and a layer at
+    x = box exactly falls outside the grid (floor(box * inv_cell) = grid)."""
+    p = configs._plain_params((0.5, 0.5, 0.5), (16, 16, 16))
+    p.h = np.float32(1.0 / 32.0)
+    for a in range(3):
+        p.inv_cell[a] = np.float32(32.0)
+    p.mass = np.float32(configs.RHO0 * (1.0 / 64.0) ** 3)
+    p.particle_size = np.float32(1.0 / 128.0)
+    ij = np.stack(np.meshgrid(np.arange(20), np.arange(20), np.arange(12), indexing="ij"), -1).reshape(-1, 3)
+    layer = np.stack(np.meshgrid([32], np.arange(4, 12), np.arange(4, 12), indexing="ij"), -1).reshape(-1, 3)
+    pos = (np.concatenate([ij, layer]) / 64.0).astype(np.float32)
+    return configs.Config("exact_lattice", p, pos, np.zeros_like(pos), 1e-4, 1)
+
+
+def test_exact_cell_faces_and_pairs_at_h(gpu, oracle, oracle_omp):
+    """Keys on cell faces, pairs at exactly r = h (inside: the predicate is
+    r^2 <= h^2) and positions at exactly the box edge: neighbour sets bit
+    for bit against the oracle, then one replayed substep."""
+    cfg = exact_lattice_scene()
+    h2 = np.float32(cfg.params.h) * np.float32(cfg.params.h)
+    sc = gpu.Scene(cfg.params, cfg.n)
+    try:
+        sc.upload(cfg.pos, cfg.vel)
+        cnt, lst = sc.neighbours()
+    finally:
+        sc.close()
+    ocnt, olst = oracle.neighbours(cfg.params, cfg.pos)
+    assert np.array_equal(cnt, ocnt) and np.array_equal(lst, olst), "neighbour sets differ"
+    first = np.repeat(np.arange(cfg.n), cnt.astype(np.int64))
+    d = cfg.pos[lst.astype(np.int64)] - cfg.pos[first]
+    r2 = (d.astype(np.float64) ** 2).sum(axis=1)
+    assert np.any(r2 == float(h2)), "expected pairs at exactly r = h"
+    g = gpu_substep(gpu, cfg, cfg.pos, cfg.vel, np.zeros_like(cfg.pos))
+    o = oracle_substep(oracle_omp, cfg, cfg.pos, cfg.vel, np.zeros_like(cfg.pos))
+    assert np.any(o["keys"] == 0xFFFFFFFF) and np.any(cfg.pos[:, 0] * np.float32(32.0) == np.float32(8.0))
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "exact lattice")
+
+
 def test_replay_dense_rows(gpu, oracle_omp):
     """A block at h/4 spacing: stencil rows of more than 64 slots (scanned in
     full) inside it, mask-walked rows of at most 64 slots at its faces."""
