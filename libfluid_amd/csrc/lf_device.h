@@ -124,6 +124,21 @@ __device__ __forceinline__ uint32_t *move_slot(uint32_t *moves, uint32_t scene, 
     return moves + ((size_t)scene * kMoveSlots + k) * kMoveStride;
 }
 constexpr double kSkinMin = 0.01, kSkinMax = 0.1;
+
+// Near-isolated marks: one byte per kNearBlock slots of cell order.  A wave
+// of the normal, force or XSPH pass takes the FP64-capable walk only when a
+// slot of its lanes' nine rows [b[r], e[r]) lies in a marked block; every
+// pair that walk treats specially has a particle the density pass marked.
+constexpr uint32_t kNearBlock = 64u;
+__device__ __forceinline__ bool rows_near(const uint8_t *__restrict__ near, const uint32_t b[9],
+                                          const uint32_t e[9]) {
+    uint32_t any = 0u;
+#pragma unroll
+    for (int r = 0; r < 9; ++r)
+        if (b[r] < e[r])
+            for (uint32_t k = b[r] / kNearBlock; k <= (e[r] - 1u) / kNearBlock; ++k) any |= near[k];
+    return any != 0u;
+}
 // A particle's move in the substep, v dt + a dt^2 / 2, is bounded without
 // reading a by |v| dt + acc_limit dt^2 / 2: the integrator clamps |a| to
 // acc_limit (a larger uploaded acceleration is caught by the far-move check).

@@ -19,12 +19,15 @@ struct Launch {
 // the scene owning slots [512 c, 512 c + 512) (kNoScene if none); nslots is
 // the pool's slot count (a multiple of 512), the stride of its per-slot
 // planes.  Stepping launches cover slots [0, nlaunch) (a multiple of 512, at
-// most nslots), which hold every active scene.
+// most nslots), which hold every active scene.  near holds one byte per
+// kNearBlock slots of cell order: cleared by the gather pass, set by the
+// density pass where a near-isolated particle sits.
 struct PoolView {
     const Consts *cs;
     const uint32_t *chunk;
     uint32_t nslots;
     uint32_t nlaunch;
+    uint8_t *near;
 };
 
 uint32_t scan_tiles(uint32_t m);

@@ -217,7 +217,8 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
                                                    const float4 *__restrict__ V,
                                                    float4 *__restrict__ Ps, float4 *__restrict__ Vs,
                                                    uint32_t *__restrict__ order, float *__restrict__ soa,
-                                                   uint32_t nslots, uint32_t *__restrict__ moves,
+                                                   uint32_t nslots, uint8_t *__restrict__ near,
+                                                   uint32_t *__restrict__ moves,
                                                    uint32_t *__restrict__ err) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
         Vs[f] = v;
         order[f] = src;
         q = skin_quantum(*cp, v);
+        if (s % kNearBlock == 0u) near[s / kNearBlock] = 0u;  // marked again by k_density
     }
     // The scene's largest move of the substep (XSPH skin): reduced over the
     // workgroup, then one atomicMax into the slot of the workgroup's index.
@@ -457,7 +459,7 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
                                                 float4 *__restrict__ D,
                                                 float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
                                                 RowMasks MS, uint32_t *__restrict__ flags,
-                                                const uint32_t *__restrict__ moves) {
+                                                const uint32_t *__restrict__ moves, uint8_t *__restrict__ near) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
@@ -584,7 +586,10 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     // Every pair the later passes treat as near-isolated (1/rho_j or 1/rho_i
     // above fragile_ir; rho_i + rho_j below fragile_sr, which needs one
     // density below fragile_sr / 2) has a particle that sets the flag here.
-    if (ir > c.fragile_ir || rho < 0.5f * c.fragile_sr) set_flag(flags, c.scene, kFragileByte, 1u);
+    if (ir > c.fragile_ir || rho < 0.5f * c.fragile_sr) {
+        set_flag(flags, c.scene, kFragileByte, 1u);
+        near[i / kNearBlock] = 1u;
+    }
     D[i] = make_float4(rho, press, cs_i, ir);
     Q[i] = make_float4(pi.x, pi.y, pi.z, rho > c.eps ? ir : -1.0f);
     Pr[i] = press;
@@ -618,7 +623,8 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                 uint32_t nslots, const float4 *__restrict__ Q,
                                                 const uint32_t *__restrict__ start, RowMasks M, int flat, int all,
-                                                const uint32_t *__restrict__ flags, float4 *__restrict__ N) {
+                                                const uint32_t *__restrict__ flags,
+                                                const uint8_t *__restrict__ near, float4 *__restrict__ N) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp || (cp->lazy_normal && !all)) return;
@@ -641,7 +647,7 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
             tab, b, e, M, c.use_masks, flat, nslots, i, [&](uint32_t j) { return at32(Q, j); },
             [&](const float4 &q, uint32_t, bool ok) { normal_term<F>(c, pi, q, dist2(pi, q), ok, sx, sy, sz); });
     };
-    if (__builtin_expect(flag_set(flag_word(flags, c.scene), kFragileByte), 0))
+    if (__builtin_expect(flag_set(flag_word(flags, c.scene), kFragileByte), 0) && lanes(rows_near(near, b, e)))
         walk.template operator()<true>();
     else
         walk.template operator()<false>();
@@ -667,7 +673,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
                                                float4 *__restrict__ Lout, float4 *__restrict__ X,
                                                float4 *__restrict__ Fdbg, uint32_t *__restrict__ flags,
-                                               const uint32_t *__restrict__ moves) {
+                                               const uint32_t *__restrict__ moves, const uint8_t *__restrict__ near) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
     if (!cp) return;
@@ -801,7 +807,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                     walk.template operator()<false, false, F>();
             }
         };
-        if (flag_set(sw, kFragileByte))
+        if (flag_set(sw, kFragileByte) && lanes(rows_near(near, b, e)))
             terms.template operator()<true>();
         else
             terms.template operator()<false>();
@@ -906,6 +912,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
                                              const float4 *__restrict__ X, const float4 *__restrict__ Vh,
                                              const float4 *__restrict__ Lw, const uint32_t *__restrict__ start,
                                              RowMasks MS, int flat, const uint32_t *__restrict__ flags,
+                                             const uint8_t *__restrict__ near,
                                              float4 *__restrict__ Vout) {
     uint32_t slot0;
     const Consts *cp = block_scene<BS>(cs, chunk, slot0);
@@ -965,7 +972,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
                 sz += w * (vj.z - vi.z);
             });
     };
-    if (__builtin_expect(flag_set(fl, kFragileByte), 0))
+    if (__builtin_expect(flag_set(fl, kFragileByte), 0) && lanes(rows_near(near, b, e)))
         walk.template operator()<true>();
     else
         walk.template operator()<false>();
@@ -1249,7 +1256,7 @@ void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
                    float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
     LF_GO(L, k_gather, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          Ps, Vs, order, soa, pv.nslots, moves, err);
+          Ps, Vs, order, soa, pv.nslots, pv.near, moves, err);
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
@@ -1257,7 +1264,7 @@ void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *P
                     uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_density<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
-              MS, flags, moves);
+              MS, flags, moves, pv.near);
     });
 }
 
@@ -1265,7 +1272,7 @@ void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool 
                     const uint32_t *start, RowMasks M, const uint32_t *flags, float4 *N) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_normals<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Q, start, M, flat,
-              all ? 1 : 0, flags, N);
+              all ? 1 : 0, flags, pv.near, N);
     });
 }
 
@@ -1275,7 +1282,7 @@ void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const 
                    float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, A, Lw, order,
-              D, N, start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves);
+              D, N, start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves, pv.near);
     });
 }
 
@@ -1284,7 +1291,7 @@ void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const fl
                  float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_xsph<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
-              flat, flags, Vout);
+              flat, flags, pv.near, Vout);
     });
 }
 

@@ -144,7 +144,7 @@ class Pool {
     template <class Fn>
     void timed(int kind, Fn &&launch);
     void drop_graph();
-    PoolView view() const { return PoolView{d_cs_, d_chunk_, nslots_, span_slots_}; }
+    PoolView view() const { return PoolView{d_cs_, d_chunk_, nslots_, span_slots_, near_}; }
     void set_spans();
 
     int device_ = 0;
@@ -170,6 +170,7 @@ class Pool {
     uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr, *err_ = nullptr;
     uint32_t *flags_ = nullptr;  // kSceneWords per scene: the substep's flag bytes
     uint32_t *moves_ = nullptr;  // kMoveSlots * kMoveStride per scene: the substep's largest move
+    uint8_t *near_ = nullptr;    // one byte per kNearBlock slots: near-isolated marks
     Consts *d_cs_ = nullptr;
     uint32_t *d_chunk_ = nullptr;
 

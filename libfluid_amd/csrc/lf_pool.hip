@@ -116,7 +116,7 @@ void Pool::release() {
     drop_graph();
     void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, soa_, M_.w32, M_.w64, MS_.w32,
                     MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, flags_, moves_,
-                    d_cs_, d_chunk_};
+                    near_, d_cs_, d_chunk_};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     for (auto &r : prof_live_) {
@@ -181,6 +181,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (!rc) rc = dalloc(&err_, 1);
     if (!rc) rc = dalloc(&flags_, (size_t)kSceneWords * scene_cap);
     if (!rc) rc = dalloc(&moves_, (size_t)kMoveSlots * kMoveStride * scene_cap);
+    if (!rc) rc = dalloc(&near_, n / kNearBlock);
     if (!rc) rc = dalloc(&d_cs_, scene_cap);
     if (!rc) rc = dalloc(&d_chunk_, h_chunk_.size());
     for (int k = 0; k < kTableRing && !rc; ++k) {
@@ -198,6 +199,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         if (e == hipSuccess) e = hipMemsetAsync(flags_, 0, sizeof(uint32_t) * kSceneWords * scene_cap, stream_);
         if (e == hipSuccess)
             e = hipMemsetAsync(moves_, 0, sizeof(uint32_t) * kMoveSlots * kMoveStride * scene_cap, stream_);
+        if (e == hipSuccess) e = hipMemsetAsync(near_, 0, n / kNearBlock, stream_);
         if (e != hipSuccess) rc = fail(LF_ERR_HIP, "pool: %s", hipGetErrorString(e));
     }
     if (!rc) rc = push_tables();

@@ -284,7 +284,8 @@ int lf_batch_footprint(uint32_t max_particles, uint32_t max_cells, uint32_t max_
     // move table (kMoveSlots words one cache line apart).
     const uint64_t per_slot = 13 * 16 + 4 + 3 * 4 + 9 * (4 + 8) * 2 + 4 * 4;
     const uint64_t per_scene = sizeof(lf::Consts) + 4 * lf::kSceneWords + 4 * lf::kMoveSlots * lf::kMoveStride;
-    *bytes = slots * per_slot + (uint64_t)max_cells * 8 + (uint64_t)max_scenes * per_scene + slots / lf::kChunk * 4;
+    *bytes = slots * per_slot + (uint64_t)max_cells * 8 + (uint64_t)max_scenes * per_scene + slots / lf::kChunk * 4 +
+             slots / lf::kNearBlock;
     return LF_OK;
 }
 

@@ -157,12 +157,13 @@ def test_checkpoint_valid_file_reaches_scene_creation(tmp_path):
 def test_batch_footprint():
     """Bytes a batch allocates: 456 per slot (slots rounded up to 512), 8 per
     bucket, per scene its table entry, flag word and 2 KB move table (16
-    slots a cache line apart), 4 per 512-slot chunk."""
+    slots a cache line apart), per 512-slot chunk 4 for its map entry and 8
+    near-isolated mark bytes (one per 64 slots)."""
     b1 = lf.Batch.footprint(1, 10, 1)
     b2 = lf.Batch.footprint(512, 10, 1)
     assert b1 == b2  # one chunk either way
     b3 = lf.Batch.footprint(1024, 10, 1)
-    assert b3 - b2 == 512 * 456 + 4
+    assert b3 - b2 == 512 * 456 + 4 + 8
     assert lf.Batch.footprint(512, 20, 1) - b2 == 80
     per_scene = lf.Batch.footprint(512, 10, 2) - b2
     assert 2048 + 200 < per_scene < 2048 + 1024
