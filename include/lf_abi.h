@@ -32,7 +32,7 @@ enum lf_status {
     LF_ERR_HIP = -2,      /* HIP runtime failure */
     LF_ERR_ALLOC = -3,    /* device or host allocation failed */
     LF_ERR_STATE = -4,    /* call not valid in the scene's current state */
-    LF_ERR_OVERFLOW = -5, /* a cell held more particles than the in-cell ranking supports */
+    LF_ERR_OVERFLOW = -5, /* kept for source compatibility; never returned (cells of any occupancy are sorted) */
     LF_ERR_FULL = -6      /* a batch has no room for another scene */
 };
 
@@ -152,6 +152,25 @@ int lf_set_debug(lf_scene *s, int flags);
  * NULL the ids of each set, ascending, concatenated in id order.
  * *total receives the number of pairs; cap bounds `list`. */
 int lf_neighbours(lf_scene *s, uint32_t *counts, uint32_t *list, uint64_t cap, uint64_t *total);
+
+/* Host-side check of the flattened neighbour walk's table (no counterpart
+ * in the reference): for one particle's nine stencil rows, masks[r] (high
+ * word zero for a row of at most 32 slots) and first_slot[r], builds the
+ * table the normals, forces and XSPH passes walk with the kernels' own code:
+ * the non-empty 32-slot pseudo-rows in row order, (entry_mask[k],
+ * entry_base[k]) for k < min(*count, 16); *count is the number of non-empty
+ * pseudo-rows (above 16 the kernels walk row by row instead).  The entry
+ * arrays hold 16 elements. */
+int lf_debug_flat_table(const uint64_t masks[9], const uint32_t first_slot[9], uint32_t *entry_mask,
+                        uint32_t *entry_base, uint32_t *count);
+/* Test hooks for the allocation-failure paths (no counterpart in the
+ * reference): after lf_debug_fail_alloc(k) the library's (k+1)-th device
+ * allocation from now fails with LF_ERR_ALLOC (k < 0: none fails);
+ * lf_debug_live_allocs reports how many device allocations the library
+ * holds, so a test can check that a failed create releases each exactly
+ * once. */
+int lf_debug_fail_alloc(int64_t nth);
+int lf_debug_live_allocs(int64_t *count);
 
 /* ---- timing ------------------------------------------------------------ */
 /* Kernel classes timed by lf_profile_*. */

@@ -192,6 +192,9 @@ def _load():
         "lf_server_job_status": (C.c_int, [C.c_void_p, C.c_uint64, P(C.c_int), u32p]),
         "lf_server_job_result": (C.c_int, [C.c_void_p, C.c_uint64, f32p, f32p, f32p, f32p, P(C.c_double)]),
         "lf_server_get_stats": (C.c_int, [C.c_void_p, P(LfServerStats)]),
+        "lf_debug_flat_table": (C.c_int, [P(C.c_uint64), u32p, u32p, u32p, u32p]),
+        "lf_debug_fail_alloc": (C.c_int, [C.c_int64]),
+        "lf_debug_live_allocs": (C.c_int, [P(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -217,7 +220,8 @@ EXPORTED = (
     "lf_server_job_status lf_server_job_result lf_server_get_stats lf_server_take_finished lf_server_configure "
     "lf_checkpoint_save lf_checkpoint_load lf_setup_jittered_lattice lf_setup_fluid_block lf_export_geo_async "
     "lf_export_wait lf_batch_checkpoint_save lf_batch_checkpoint_load lf_batch_setup_jittered_lattice "
-    "lf_batch_setup_fluid_block lf_batch_export_geo_async lf_batch_export_wait"
+    "lf_batch_setup_fluid_block lf_batch_export_geo_async lf_batch_export_wait "
+    "lf_debug_flat_table lf_debug_fail_alloc lf_debug_live_allocs"
 ).split()
 
 
@@ -238,6 +242,29 @@ def uptr(a: np.ndarray | None):
         return None
     assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+FLAT_ENTRIES = 16  # entries of the flattened walk's table
+
+
+def flat_table(masks, first_slots):
+    """The flattened walk's table for one particle's nine rows, built by the
+    kernels' own code on the host: (entry masks, entry first slots, count)."""
+    m = np.ascontiguousarray(masks, np.uint64)
+    b = np.ascontiguousarray(first_slots, np.uint32)
+    assert m.shape == (9,) and b.shape == (9,)
+    em = np.zeros(FLAT_ENTRIES, np.uint32)
+    eb = np.zeros(FLAT_ENTRIES, np.uint32)
+    cnt = C.c_uint32(0)
+    check(lib.lf_debug_flat_table(m.ctypes.data_as(C.POINTER(C.c_uint64)), uptr(b), uptr(em), uptr(eb),
+                                  C.byref(cnt)), "lf_debug_flat_table")
+    return em, eb, int(cnt.value)
+
+
+def live_allocs() -> int:
+    c = C.c_int64(0)
+    check(lib.lf_debug_live_allocs(C.byref(c)), "lf_debug_live_allocs")
+    return c.value
 
 
 def device_count() -> int:

@@ -181,3 +181,14 @@ def spray_only(n: int = 262_144, seed: int = 7) -> Config:
 
 
 CONFIGS = {"c1": dambreak_c1, "c2": column_c2, "c3": spray_c3, "double_dam": double_dam}
+
+
+def cloud(ppc: float = 18.0, side: float = 0.2, seed: int = 5) -> Config:
+    """A uniform random cloud of about ppc particles per cell in a cube of
+    the given side inside the C1 box: stencil rows of 33 to 64 slots whose
+    neighbours fall unevenly into their two 32-slot halves, so some
+    particles' flattened-walk tables hold exactly 16 entries."""
+    p = _plain_params((1.0, 0.5, 0.5), (50, 25, 25))
+    n = int(ppc * (side / H) ** 3)
+    pos = uniform_points(n, (0.3, 0.1, 0.1), (0.3 + side, 0.1 + side, 0.1 + side), seed)
+    return Config(f"cloud_{n}", p, pos, np.zeros_like(pos), 1e-4, 20)

@@ -11,7 +11,13 @@ constexpr int kBlock = 256;                   // threads per workgroup of the fi
 constexpr uint32_t kChunk = 512;              // particle-slot granularity of scene ranges (largest block)
 constexpr uint32_t kNoScene = 0xFFFFFFFFu;    // chunk owned by no scene
 constexpr uint32_t kMaxParticles = 1u << 28;  // pool slots: 16-byte records addressed with 32-bit offsets
-constexpr uint32_t kRankFixMax = 4096;        // largest bucket the in-cell id ranking handles
+// Stable placement by (cell, id): a bucket of at most kRankDirectMax
+// particles is ranked by id directly (each member counts the smaller ids,
+// O(size^2) per bucket); a larger one is sorted by id as a segment
+// (k_sort_segments: LSD radix sort, O(size) per pass).
+constexpr uint32_t kRankDirectMax = 512;
+constexpr int kSegThreads = 1024;  // workgroup of k_sort_segments: one segment at a time
+constexpr int kSegBlocks = 128;    // its workgroups (segments are taken in a grid stride)
 constexpr int kScanPerThread = 4;             // cell-count scan: entries per thread (one 16-byte load)
 constexpr int kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kScanDirectTiles = 2048;    // above this many tiles the tile prefixes get their own pass
@@ -84,6 +90,34 @@ __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.
 __device__ __forceinline__ float dist2(float4 a, float4 b) {
     float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
     return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+// Table of one lane's flattened neighbour walk: the non-empty pseudo-rows of
+// its nine stencil rows, in row order, as (mask word, first slot).  A row of
+// at most 32 slots is one pseudo-row; a row of 33..64 slots is two, the low
+// and high words of its 64-bit mask, the second starting 32 slots later.
+// mw[r] is row r's mask (high word zero for a row of at most 32 slots), b[r]
+// its first slot.  put(k, mask, base) stores entry k, for k < kFlatEntries
+// only; empty pseudo-rows are skipped without a store, so entries [0, cnt)
+// are exactly the non-empty ones.  Returns cnt, the number of non-empty
+// pseudo-rows (above kFlatEntries the table is incomplete and must not be
+// walked).  Shared by the kernels and a host-side check of the table.
+constexpr int kFlatEntries = 16;
+template <class Put>
+__host__ __device__ __forceinline__ uint32_t flat_table(const uint64_t mw[9], const uint32_t b[9], Put &&put) {
+    uint32_t cnt = 0u;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+#pragma unroll
+        for (uint32_t half = 0u; half < 2u; ++half) {
+            const uint32_t m = (uint32_t)(mw[r] >> (32u * half));
+            if (m != 0u) {
+                if (cnt < (uint32_t)kFlatEntries) put(cnt, m, b[r] + 32u * half);
+                ++cnt;
+            }
+        }
+    }
+    return cnt;
 }
 
 // Per-scene words of the substep, kSceneWords per scene.

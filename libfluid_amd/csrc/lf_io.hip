@@ -40,7 +40,7 @@ int Pool::save_state(uint32_t id, std::vector<float> &out) {
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out.data(), d, 12 * n * sizeof(float), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d);
+    dev_free(d);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "save_state: %s", hipGetErrorString(e));
     return LF_OK;
 }
@@ -57,7 +57,7 @@ int Pool::load_state(uint32_t id, const float *in, uint64_t steps) {
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d);
+    dev_free(d);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "load_state: %s", hipGetErrorString(e));
     s->identity = true;
     s->fields = false;
@@ -110,7 +110,7 @@ int Pool::setup_block(uint32_t id, uint32_t id0, uint32_t n, const float mn[3], 
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d);
+    dev_free(d);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "setup_block: %s", hipGetErrorString(e));
     return LF_OK;
 }
@@ -191,8 +191,8 @@ void Pool::stop_exporter() {
     exporter_->cv.notify_all();
     if (exporter_->worker.joinable()) exporter_->worker.join();
     for (auto &b : exporter_->bufs) {
-        if (b.dev) (void)hipFree(b.dev);
-        if (b.ids) (void)hipFree(b.ids);
+        dev_free(b.dev);
+        dev_free(b.ids);
         if (b.host) (void)hipHostFree(b.host);
         if (b.snap) (void)hipEventDestroy(b.snap);
         if (b.copied) (void)hipEventDestroy(b.copied);
@@ -235,17 +235,19 @@ int Pool::export_geo_async(uint32_t id, const char *filebase, uint32_t cnt, int 
         if (int rc = export_wait()) return rc;
         Exporter::Buf b;
         b.cap = n;
-        hipError_t e = hipMalloc((void **)&b.dev, 12 * n * sizeof(float));
-        if (e == hipSuccess) e = hipMalloc((void **)&b.ids, n * sizeof(uint32_t));
-        if (e == hipSuccess) e = hipHostMalloc((void **)&b.host, 12 * n * sizeof(float), hipHostMallocDefault);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b.snap, hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreateWithFlags(&b.copied, hipEventDisableTiming);
-        if (e != hipSuccess) {
-            if (b.dev) (void)hipFree(b.dev);
-            if (b.ids) (void)hipFree(b.ids);
+        int rc = dalloc(&b.dev, 12 * n);
+        if (!rc) rc = dalloc(&b.ids, n);
+        hipError_t e = hipSuccess;
+        if (!rc) e = hipHostMalloc((void **)&b.host, 12 * n * sizeof(float), hipHostMallocDefault);
+        if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b.snap, hipEventDisableTiming);
+        if (!rc && e == hipSuccess) e = hipEventCreateWithFlags(&b.copied, hipEventDisableTiming);
+        if (rc || e != hipSuccess) {
+            dev_free(b.dev);
+            dev_free(b.ids);
             if (b.host) (void)hipHostFree(b.host);
             if (b.snap) (void)hipEventDestroy(b.snap);
             if (b.copied) (void)hipEventDestroy(b.copied);
+            if (rc) return rc;
             return fail(LF_ERR_ALLOC, "export_geo_async: %s", hipGetErrorString(e));
         }
         std::lock_guard<std::mutex> lk(x.m);

@@ -34,19 +34,24 @@ uint32_t scan_tiles(uint32_t m);
 // Block sizes the neighbour passes are built for (128, 256, 512).
 bool valid_block_size(int bs);
 
+// Also clears the count of large-bucket segments (nsegs).
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
-                 uint32_t *counts, uint32_t *flags, uint32_t *moves);
+                 uint32_t *counts, uint32_t *flags, uint32_t *moves, uint32_t *nsegs);
 // Exclusive scan of counts[0, m) into start[0, m] (start[m] = total), which
 // also clears the counts; bsum holds scan_tiles(m) + 1 words.
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
+// Provisional slots by atomic rank; then every bucket of more than
+// kRankDirectMax particles is sorted by id in place (its segment listed in
+// segs, nsegs of them; tmp: one uint2 per pool slot, scratch).
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
-                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id);
+                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id, uint4 *segs, uint32_t *nsegs,
+                    uint2 *tmp);
 // Copies positions and velocities to cell order; order[f] receives the state
 // slot of the particle in slot f (acceleration and low position words stay
 // in the state arrays, read there by the force pass).
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
-                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err);
+                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves);
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
                     const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
                     uint32_t *flags, const uint32_t *moves);

@@ -49,7 +49,9 @@ namespace lf {
 __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                  const float4 *__restrict__ P, uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts,
-                                                 uint32_t *__restrict__ flags, uint32_t *__restrict__ moves) {
+                                                 uint32_t *__restrict__ flags, uint32_t *__restrict__ moves,
+                                                 uint32_t *__restrict__ nsegs) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nsegs = 0u;  // large buckets of the substep, listed by k_scatter
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
@@ -185,26 +187,148 @@ __device__ __forceinline__ uint32_t slot_shift(const Consts &c, const uint32_t *
     return c.first - start[c.cell_base];
 }
 
-// 3. provisional scatter: slot = bucket start + atomic rank.
+// 3. provisional scatter: slot = bucket start + atomic rank.  The particle
+// with rank 0 of a bucket of more than kRankDirectMax particles lists the
+// bucket's slot range as a segment for k_sort_segments: (first slot,
+// size, bits of the scene's largest id).
 __global__ __launch_bounds__(kBlock) void k_scatter(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                     const float4 *__restrict__ P, const uint32_t *__restrict__ keys,
                                                     const uint32_t *__restrict__ ranks,
                                                     const uint32_t *__restrict__ start,
-                                                    uint32_t *__restrict__ slot_src, uint32_t *__restrict__ slot_id) {
+                                                    uint32_t *__restrict__ slot_src, uint32_t *__restrict__ slot_id,
+                                                    uint4 *__restrict__ segs, uint32_t *__restrict__ nsegs) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
     const uint32_t i = slot0 + threadIdx.x;
     if (i >= cp->first + cp->count) return;
-    const uint32_t s = start[keys[i]] + slot_shift(*cp, start) + ranks[i];
+    const uint32_t b = keys[i], r = ranks[i];
+    const uint32_t lo = start[b] + slot_shift(*cp, start);
+    const uint32_t s = lo + r;
     slot_src[s] = i;
     slot_id[s] = __float_as_uint(P[i].w);
+    if (r == 0u) {
+        const uint32_t size = start[b + 1] - start[b];
+        if (size > kRankDirectMax) {
+            const uint32_t bits = cp->count > 1u ? 32u - (uint32_t)__clz(cp->count - 1u) : 1u;
+            segs[atomicAdd(nsegs, 1u)] = make_uint4(lo, size, bits, 0u);
+        }
+    }
+}
+
+// 3b. segments (buckets of more than kRankDirectMax particles) sorted by id
+// in place: (slot_id, slot_src) of slots [lo, lo + size).  One workgroup per
+// segment (grid stride over the list k_scatter wrote), LSD radix sort by 8
+// bits of the id per pass, through tmp (same slots) and back.  A pass
+// counts the digits, then places the segment tile by tile (kSegThreads
+// elements, in slot order): a member's new place is the digit's running
+// base, plus that digit's count in the tile's earlier waves, plus its rank
+// among the earlier lanes of its wave with the same digit (lanes matched by
+// eight ballots), so every pass is stable.  Ids are unique within a scene.
+__device__ __forceinline__ uint32_t seg_digit(uint32_t id, uint32_t shift) { return (id >> shift) & 0xFFu; }
+
+__global__ __launch_bounds__(kSegThreads) void k_sort_segments(const uint4 *__restrict__ segs,
+                                                               const uint32_t *__restrict__ nsegs,
+                                                               uint32_t *__restrict__ slot_id,
+                                                               uint32_t *__restrict__ slot_src,
+                                                               uint2 *__restrict__ tmp) {
+    constexpr int kWaves = kSegThreads / 64;
+    __shared__ uint32_t base[256];           // running start of each digit in the destination
+    __shared__ uint32_t wcnt[kWaves][256];   // per wave: digit counts, then their prefix over waves
+    __shared__ uint32_t scan_tot[4];
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t nseg = *nsegs;
+    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
+        const uint4 g = segs[sg];
+        const uint32_t lo = g.x, n = g.y, passes = (g.z + 7u) / 8u;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const bool from_tmp = (p & 1u) != 0u;
+            const uint32_t shift = 8u * p;
+            if (t < 256u) base[t] = 0u;
+            __syncthreads();
+            for (uint32_t k = t; k < n; k += kSegThreads) {
+                const uint32_t id = from_tmp ? tmp[lo + k].x : slot_id[lo + k];
+                atomicAdd(&base[seg_digit(id, shift)], 1u);
+            }
+            __syncthreads();
+            // exclusive scan of the 256 digit counts (waves 0-3)
+            uint32_t v = 0u, incl = 0u;
+            if (t < 256u) {
+                v = base[t];
+                incl = v;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t u = __shfl_up(incl, o, 64);
+                    if ((int)lane >= o) incl += u;
+                }
+                if (lane == 63u) scan_tot[w] = incl;
+            }
+            __syncthreads();
+            if (t < 256u) {
+                uint32_t off = 0u;
+                for (uint32_t q = 0; q < w; ++q) off += scan_tot[q];
+                base[t] = off + incl - v;
+            }
+            __syncthreads();
+            for (uint32_t tile = 0; tile < n; tile += kSegThreads) {
+                for (uint32_t q = t; q < kWaves * 256u; q += kSegThreads) (&wcnt[0][0])[q] = 0u;
+                __syncthreads();
+                const uint32_t k = tile + t;
+                const bool valid = k < n;
+                uint2 e = make_uint2(0u, 0u);
+                if (valid) e = from_tmp ? tmp[lo + k] : make_uint2(slot_id[lo + k], slot_src[lo + k]);
+                const uint32_t d = seg_digit(e.x, shift);
+                uint64_t peers = __ballot(valid);
+#pragma unroll
+                for (uint32_t bit = 0; bit < 8u; ++bit) {
+                    const bool on = ((d >> bit) & 1u) != 0u;
+                    const uint64_t m = __ballot(on);
+                    peers &= on ? m : ~m;
+                }
+                const uint32_t rank = (uint32_t)__popcll(peers & below);
+                if (valid && rank == 0u) wcnt[w][d] = (uint32_t)__popcll(peers);
+                __syncthreads();
+                uint32_t tile_total = 0u;
+                if (t < 256u) {  // prefix of digit t's counts over the waves
+                    for (uint32_t q = 0; q < (uint32_t)kWaves; ++q) {
+                        const uint32_t c = wcnt[q][t];
+                        wcnt[q][t] = tile_total;
+                        tile_total += c;
+                    }
+                }
+                __syncthreads();
+                if (valid) {
+                    const uint32_t dst = lo + base[d] + wcnt[w][d] + rank;
+                    if (from_tmp) {
+                        slot_id[dst] = e.x;
+                        slot_src[dst] = e.y;
+                    } else {
+                        tmp[dst] = e;
+                    }
+                }
+                __syncthreads();
+                if (t < 256u) base[t] += tile_total;
+                __syncthreads();
+            }
+        }
+        if (passes & 1u) {  // the last pass wrote tmp
+            for (uint32_t k = t; k < n; k += kSegThreads) {
+                const uint2 e = tmp[lo + k];
+                slot_id[lo + k] = e.x;
+                slot_src[lo + k] = e.y;
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // 4. stable placement + payload gather (sort_by_key of Points, Velocity,
 // Acceleration, fluidsystem.cu:309-313).  The final slot of a particle is
 // its bucket start plus the number of bucket members with a smaller id, so
-// the order is (key, id) whatever order the atomics handed out.  Position
+// the order is (key, id) whatever order the atomics handed out; a bucket of
+// more than kRankDirectMax particles has been sorted by id in place
+// (k_sort_segments), so there the provisional slot is the final one.  Position
 // and velocity are copied to cell order (the neighbour passes gather them);
 // the acceleration and the position's low word, which only the particle's
 // own integration reads, stay where they are: order[f] records the state
@@ -218,8 +342,7 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
                                                    float4 *__restrict__ Ps, float4 *__restrict__ Vs,
                                                    uint32_t *__restrict__ order, float *__restrict__ soa,
                                                    uint32_t nslots, uint8_t *__restrict__ near,
-                                                   uint32_t *__restrict__ moves,
-                                                   uint32_t *__restrict__ err) {
+                                                   uint32_t *__restrict__ moves) {
     uint32_t slot0;
     const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
     if (!cp) return;
@@ -230,14 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
         const uint32_t b = keys[src];
         const uint32_t shift = slot_shift(*cp, start);
         const uint32_t lo = start[b] + shift, hi = start[b + 1] + shift;
-        uint32_t f;
-        if (hi - lo <= kRankFixMax) {
+        uint32_t f = s;
+        if (hi - lo <= kRankDirectMax) {
             uint32_t r = 0;
             for (uint32_t t = lo; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
             f = lo + r;
-        } else {
-            f = s;  // bucket too large to rank: keep the provisional slot, report it
-            if (s == lo) atomicOr(err, 1u);
         }
         const float4 p = P[src], v = V[src];
         Ps[f] = p;
@@ -353,8 +473,7 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
 // lane of the wave has a row of more than 64 slots or more than kFlatEntries
 // non-empty pseudo-rows, or (need_long) when no lane of the wave has a row
 // of more than 32 slots (one pseudo-row per row: the row-by-row walk then
-// does as well without the table).
-constexpr int kFlatEntries = 16;
+// does as well without the table).  The table is filled by flat_table.
 constexpr int kForceWalkWidth = 4;  // set bits per step of the flattened force walk
 constexpr int kXsphWalkWidth = 4;   // and of the flattened XSPH walk
 constexpr int kXsphRowWidth = 4;    // set bits per step of the row-by-row XSPH walk
@@ -387,18 +506,11 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
         mw[r] = len > 32u ? v : (v & 0xFFFFFFFFull);
         fits = fits && len <= 64u;
     }
-    // An empty pseudo-row is written at the running count and overwritten by
-    // the next; entries at or past cnt are never used.
-    uint32_t cnt = 0u;
-    auto put = [&](uint32_t m, uint32_t base) {
-        tab[(cnt < (uint32_t)kFlatEntries ? cnt : 0u) * BS + t] = make_uint2(m, base);
-        cnt += m != 0u ? 1u : 0u;
-    };
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        put((uint32_t)mw[r], b[r]);
-        put((uint32_t)(mw[r] >> 32), b[r] + 32u);
-    }
+    // Entries at or past cnt are never stored and never used (the reads
+    // below that may touch them are discarded).
+    const uint32_t cnt = flat_table(mw, b, [&](uint32_t k, uint32_t m, uint32_t base) {
+        tab[k * BS + t] = make_uint2(m, base);
+    });
     if (__ballot(!fits || cnt > (uint32_t)kFlatEntries)) return false;
     const uint2 e0 = tab[t];
     uint32_t m = cnt > 0u ? e0.x : 0u, base = e0.y;
@@ -1232,8 +1344,9 @@ static inline void with_bs(int bs, F &&body) {
 bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
-                 uint32_t *counts, uint32_t *flags, uint32_t *moves) {
-    LF_GO(L, k_hash, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves);
+                 uint32_t *counts, uint32_t *flags, uint32_t *moves, uint32_t *nsegs) {
+    LF_GO(L, k_hash, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves,
+          nsegs);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
@@ -1247,16 +1360,19 @@ void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, 
 }
 
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
-                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id) {
-    LF_GO(L, k_scatter, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, start, slot_src,
-          slot_id);
+                    const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id, uint4 *segs, uint32_t *nsegs,
+                    uint2 *tmp) {
+    LF_GO(L, k_scatter, pv.nlaunch / kBlock, kBlock, L.a, nullptr, pv.cs, pv.chunk, P, keys, ranks, start, slot_src,
+          slot_id, segs, nsegs);
+    LF_GO(L, k_sort_segments, kSegBlocks, kSegThreads, nullptr, L.b, (const uint4 *)segs, (const uint32_t *)nsegs,
+          slot_id, slot_src, tmp);
 }
 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
-                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves, uint32_t *err) {
+                   float4 *Vs, uint32_t *order, float *soa, uint32_t *moves) {
     LF_GO(L, k_gather, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, slot_src, slot_id, keys, start, P, V,
-          Ps, Vs, order, soa, pv.nslots, pv.near, moves, err);
+          Ps, Vs, order, soa, pv.nslots, pv.near, moves);
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,

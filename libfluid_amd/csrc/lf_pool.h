@@ -167,7 +167,9 @@ class Pool {
     float *soa_ = nullptr;  // sorted positions as three planes of nslots floats (x, y, z)
     RowMasks M_{nullptr, nullptr}, MS_{nullptr, nullptr};
     uint32_t *keys_ = nullptr, *ranks_ = nullptr, *slot_src_ = nullptr, *slot_id_ = nullptr;
-    uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr, *err_ = nullptr;
+    uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr;
+    uint4 *segs_ = nullptr;      // buckets of more than kRankDirectMax particles: (first slot, size, id bits)
+    uint32_t *nsegs_ = nullptr;  // how many segs_ the substep listed
     uint32_t *flags_ = nullptr;  // kSceneWords per scene: the substep's flag bytes
     uint32_t *moves_ = nullptr;  // kMoveSlots * kMoveStride per scene: the substep's largest move
     uint8_t *near_ = nullptr;    // one byte per kNearBlock slots: near-isolated marks
@@ -210,14 +212,10 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags);
 
 template <class T>
 int Pool::dalloc(T **p, size_t count) {
-    *p = nullptr;
-    if (count == 0) count = 1;
-    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return fail(LF_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-    }
-    return LF_OK;
+    void *v = nullptr;
+    const int rc = dev_alloc(&v, (count == 0 ? 1 : count) * sizeof(T));
+    *p = static_cast<T *>(v);
+    return rc;
 }
 
 }  // namespace lf

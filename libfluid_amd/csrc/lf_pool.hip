@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <numeric>
 
 #include "lf_internal.h"
@@ -23,6 +24,37 @@
     } while (0)
 
 namespace lf {
+
+namespace {
+
+std::atomic<int64_t> g_live_allocs{0};  // device allocations the library holds
+std::atomic<int64_t> g_fail_at{-1};     // allocations to go before one fails (-1: never)
+
+}  // namespace
+
+int dev_alloc(void **p, size_t bytes) {
+    *p = nullptr;
+    int64_t k = g_fail_at.load();
+    while (k >= 0 && !g_fail_at.compare_exchange_weak(k, k - 1)) {
+    }
+    if (k == 0) return fail(LF_ERR_ALLOC, "hipMalloc of %zu bytes failed: injected failure", bytes);
+    hipError_t e = hipMalloc(p, bytes == 0 ? 1 : bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(LF_ERR_ALLOC, "hipMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+    }
+    g_live_allocs.fetch_add(1);
+    return LF_OK;
+}
+
+void debug_fail_alloc(int64_t nth) { g_fail_at.store(nth < 0 ? -1 : nth); }
+int64_t debug_live_allocs() { return g_live_allocs.load(); }
+
+void dev_free_raw(void *p) {
+    if (!p) return;
+    (void)hipFree(p);
+    g_live_allocs.fetch_sub(1);
+}
 
 namespace {
 
@@ -112,13 +144,26 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
 // lifetime
 
 void Pool::release() {
+    // Work still queued on the stream (setup copies of a failed init, the
+    // last substeps or exports) may use any of the buffers below.
+    if (stream_) (void)hipStreamSynchronize(stream_);
     stop_exporter();
     drop_graph();
-    void *bufs[] = {P0_, P1_, V0_, V1_, V2_, A0_, A1_, L0_, L1_, D_, N_, F_, Q_, X_, Pr_, soa_, M_.w32, M_.w64, MS_.w32,
-                    MS_.w64, keys_, ranks_, slot_src_, slot_id_, counts_, start_, bsum_, err_, flags_, moves_,
-                    near_, d_cs_, d_chunk_};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
+    RowMasks *masks[] = {&M_, &MS_};
+    for (RowMasks *m : masks) {
+        dev_free(m->w32);
+        dev_free(m->w64);
+    }
+    float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &F_, &Q_, &X_};
+    for (float4 **b : f4) dev_free(*b);
+    dev_free(Pr_);
+    dev_free(soa_);
+    uint32_t **u32[] = {&keys_, &ranks_, &slot_src_, &slot_id_, &counts_, &start_, &bsum_, &nsegs_, &flags_,
+                        &moves_, &d_chunk_};
+    for (uint32_t **b : u32) dev_free(*b);
+    dev_free(segs_);
+    dev_free(near_);
+    dev_free(d_cs_);
     for (auto &r : prof_live_) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -133,15 +178,13 @@ void Pool::release() {
         pin_cs_[k] = nullptr;
         pin_chunk_[k] = nullptr;
         tables_ev_[k] = nullptr;
+        tables_ev_pending_[k] = false;
     }
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
 }
 
-Pool::~Pool() {
-    if (stream_) (void)hipStreamSynchronize(stream_);
-    release();
-}
+Pool::~Pool() { release(); }
 
 int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (slot_cap == 0 || bucket_cap == 0 || scene_cap == 0) return fail(LF_ERR_INVALID, "pool: zero capacity");
@@ -178,7 +221,8 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (!rc) rc = dalloc(&counts_, (size_t)nbuckets_);
     if (!rc) rc = dalloc(&start_, (size_t)nbuckets_ + 1);
     if (!rc) rc = dalloc(&bsum_, (size_t)ntiles_ + 1);
-    if (!rc) rc = dalloc(&err_, 1);
+    if (!rc) rc = dalloc(&nsegs_, 1);
+    if (!rc) rc = dalloc(&segs_, n / (kRankDirectMax + 1) + 1);  // each segment holds > kRankDirectMax slots
     if (!rc) rc = dalloc(&flags_, (size_t)kSceneWords * scene_cap);
     if (!rc) rc = dalloc(&moves_, (size_t)kMoveSlots * kMoveStride * scene_cap);
     if (!rc) rc = dalloc(&near_, n / kNearBlock);
@@ -195,7 +239,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         // Counts start at zero; afterwards every scan clears them.
         e = hipMemsetAsync(counts_, 0, sizeof(uint32_t) * nbuckets_, stream_);
         if (e == hipSuccess) e = hipMemsetAsync(start_, 0, sizeof(uint32_t) * (nbuckets_ + 1), stream_);
-        if (e == hipSuccess) e = hipMemsetAsync(err_, 0, sizeof(uint32_t), stream_);
+        if (e == hipSuccess) e = hipMemsetAsync(nsegs_, 0, sizeof(uint32_t), stream_);
         if (e == hipSuccess) e = hipMemsetAsync(flags_, 0, sizeof(uint32_t) * kSceneWords * scene_cap, stream_);
         if (e == hipSuccess)
             e = hipMemsetAsync(moves_, 0, sizeof(uint32_t) * kMoveSlots * kMoveStride * scene_cap, stream_);
@@ -492,11 +536,17 @@ bool Pool::any_separate_normals() const {
 // there).
 void Pool::enqueue_sort() {
     const PoolView pv = view();
-    timed(LF_K_HASH, [&](const Launch &L) { launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_); });
+    timed(LF_K_HASH, [&](const Launch &L) {
+        launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_, nsegs_);
+    });
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, span_buckets_, bsum_, start_); });
-    timed(LF_K_SCATTER, [&](const Launch &L) { launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_); });
+    // X_ (written by the force pass) is the segment sort's scratch.
+    timed(LF_K_SCATTER, [&](const Launch &L) {
+        launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_, segs_, nsegs_,
+                       reinterpret_cast<uint2 *>(X_));
+    });
     timed(LF_K_GATHER, [&](const Launch &L) {
-        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, P1_, V1_, ranks_, soa_, moves_, err_);
+        launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, P1_, V1_, ranks_, soa_, moves_);
     });
 }
 
@@ -622,13 +672,6 @@ int Pool::step(uint32_t substeps) {
 
 int Pool::sync() {
     LF_HIP(hipStreamSynchronize(stream_));
-    uint32_t flag = 0;
-    LF_HIP(hipMemcpy(&flag, err_, sizeof(flag), hipMemcpyDeviceToHost));
-    if (flag) {
-        LF_HIP(hipMemset(err_, 0, sizeof(uint32_t)));
-        return fail(LF_ERR_OVERFLOW, "a cell (or an out-of-grid bucket) held more than %u particles; "
-                                     "its in-cell order was not ranked by id", kRankFixMax);
-    }
     return LF_OK;
 }
 
@@ -670,7 +713,7 @@ int Pool::upload(uint32_t id, uint32_t first, uint32_t count, const float *pos, 
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d_stage);
+    dev_free(d_stage);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "upload: %s", hipGetErrorString(e));
     return LF_OK;
 }
@@ -699,7 +742,7 @@ int Pool::download(uint32_t id, int order, float *pos, float *col, float *vel, f
     uint32_t *d_ids = nullptr;
     if (int rc = dalloc(&d_out, 12 * n)) return rc;
     if (int rc = dalloc(&d_ids, n)) {
-        (void)hipFree(d_out);
+        dev_free(d_out);
         return rc;
     }
     launch_export(stream_, c, P0_, V0_, A0_, D_, order == LF_ORDER_ID, d_out, d_ids);
@@ -709,8 +752,8 @@ int Pool::download(uint32_t id, int order, float *pos, float *col, float *vel, f
     if (e == hipSuccess) e = hipMemcpyAsync(host.data(), d_out, host.size() * sizeof(float), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(hid.data(), d_ids, hid.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d_out);
-    (void)hipFree(d_ids);
+    dev_free(d_out);
+    dev_free(d_ids);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "download: %s", hipGetErrorString(e));
     for (size_t k = 0; k < n; ++k) {
         const float *h = &host[12 * k];
@@ -822,7 +865,7 @@ int Pool::neighbours(uint32_t id, uint32_t *counts, uint32_t *list, uint64_t cap
     if (e == hipSuccess) e = hipMemcpyAsync(cnt_slot.data(), d_cnt, 4 * n, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipMemcpyAsync(p1.data(), P1_ + s->first, 16 * n, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d_cnt);
+    dev_free(d_cnt);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "neighbours: %s", hipGetErrorString(e));
     std::vector<uint32_t> slot_of(n);
     for (size_t k = 0; k < n; ++k) {
@@ -843,7 +886,7 @@ int Pool::neighbours(uint32_t id, uint32_t *counts, uint32_t *list, uint64_t cap
     uint32_t *d_list = nullptr;
     if (int rc = dalloc(&d_off, n + 1)) return rc;
     if (int rc = dalloc(&d_list, (size_t)std::max<uint64_t>(pairs, 1))) {
-        (void)hipFree(d_off);
+        dev_free(d_off);
         return rc;
     }
     std::vector<uint32_t> lst(pairs);
@@ -854,8 +897,8 @@ int Pool::neighbours(uint32_t id, uint32_t *counts, uint32_t *list, uint64_t cap
     }
     if (e == hipSuccess && pairs) e = hipMemcpyAsync(lst.data(), d_list, 4 * pairs, hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    (void)hipFree(d_off);
-    (void)hipFree(d_list);
+    dev_free(d_off);
+    dev_free(d_list);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "neighbours: %s", hipGetErrorString(e));
     uint64_t o = 0;
     for (size_t pid = 0; pid < n; ++pid) {
@@ -898,7 +941,7 @@ int Pool::copy_probe(size_t bytes, int repeats, double *ms) {
     float4 *a = nullptr, *b = nullptr;
     if (int rc = dalloc(&a, bytes / 16)) return rc;
     if (int rc = dalloc(&b, bytes / 16)) {
-        (void)hipFree(a);
+        dev_free(a);
         return rc;
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -917,8 +960,8 @@ int Pool::copy_probe(size_t bytes, int repeats, double *ms) {
     if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
+    dev_free(a);
+    dev_free(b);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "copy_probe: %s", hipGetErrorString(e));
     *ms = (double)t / repeats;
     return LF_OK;
@@ -1031,7 +1074,7 @@ int Pool::autotune(uint32_t reps, double ms_out[4][3]) {
     for (int k = 0; k < 5; ++k)
         (void)hipMemcpyAsync(state(k), save + k * n, 16 * n, hipMemcpyDeviceToDevice, stream_);
     hipError_t e = hipStreamSynchronize(stream_);
-    (void)hipFree(save);
+    dev_free(save);
     scenes_ = keep;
     for (auto &s : scenes_) s.fields = false;  // the working data are the timing runs
     if (!rc && e != hipSuccess) rc = fail(LF_ERR_HIP, "autotune: %s", hipGetErrorString(e));

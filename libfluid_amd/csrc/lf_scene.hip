@@ -166,6 +166,28 @@ int lf_neighbours(lf_scene *s, uint32_t *counts, uint32_t *list, uint64_t cap, u
     return s->pool.neighbours(s->id, counts, list, cap, total);
 }
 
+int lf_debug_flat_table(const uint64_t masks[9], const uint32_t first_slot[9], uint32_t *entry_mask,
+                        uint32_t *entry_base, uint32_t *count) {
+    if (!masks || !first_slot || !entry_mask || !entry_base || !count)
+        return fail(LF_ERR_INVALID, "lf_debug_flat_table: null argument");
+    *count = lf::flat_table(masks, first_slot, [&](uint32_t k, uint32_t m, uint32_t base) {
+        entry_mask[k] = m;
+        entry_base[k] = base;
+    });
+    return LF_OK;
+}
+
+int lf_debug_fail_alloc(int64_t nth) {
+    lf::debug_fail_alloc(nth);
+    return LF_OK;
+}
+
+int lf_debug_live_allocs(int64_t *count) {
+    if (!count) return fail(LF_ERR_INVALID, "null count");
+    *count = lf::debug_live_allocs();
+    return LF_OK;
+}
+
 int lf_profile_enable(lf_scene *s, int mask) {
     if (!s) return fail(LF_ERR_INVALID, "null scene");
     return s->pool.profile_enable((uint32_t)mask);

@@ -120,9 +120,9 @@ struct lf_server {
             (void)hipEventDestroy(s.b);
         }
         for (auto e : ev_free) (void)hipEventDestroy(e);
-        if (d_in) (void)hipFree(d_in);
-        if (d_out) (void)hipFree(d_out);
-        if (d_ids) (void)hipFree(d_ids);
+        lf::dev_free(d_in);
+        lf::dev_free(d_out);
+        lf::dev_free(d_ids);
     }
 };
 
@@ -171,18 +171,15 @@ int ensure_device(lf_server *s, uint32_t n, bool out) {
     LF_HIP(hipStreamSynchronize(s->pool.stream()));
     const size_t want = std::max<size_t>(n, 2 * cap);
     if (out) {
-        if (s->d_out) (void)hipFree(s->d_out);
-        if (s->d_ids) (void)hipFree(s->d_ids);
-        s->d_out = nullptr;
-        s->d_ids = nullptr;
+        lf::dev_free(s->d_out);
+        lf::dev_free(s->d_ids);
         cap = 0;
-        LF_HIP(hipMalloc((void **)&s->d_out, want * 12 * sizeof(float)));
-        LF_HIP(hipMalloc((void **)&s->d_ids, want * sizeof(uint32_t)));
+        if (int rc = lf::dev_alloc((void **)&s->d_out, want * 12 * sizeof(float))) return rc;
+        if (int rc = lf::dev_alloc((void **)&s->d_ids, want * sizeof(uint32_t))) return rc;
     } else {
-        if (s->d_in) (void)hipFree(s->d_in);
-        s->d_in = nullptr;
+        lf::dev_free(s->d_in);
         cap = 0;
-        LF_HIP(hipMalloc((void **)&s->d_in, want * 9 * sizeof(float)));
+        if (int rc = lf::dev_alloc((void **)&s->d_in, want * 9 * sizeof(float))) return rc;
     }
     cap = want;
     return LF_OK;

@@ -269,3 +269,49 @@ def test_python_mirror_has_reference_api():
     header = open(os.path.join(ROOT, "include", "fluidsystem.h")).read()
     for name in public + protected:
         assert re.search(rf"\b{name}\s*\(", header), name
+
+
+def _flat_reference(masks, first):
+    """The table the flattened walk must see: every non-empty 32-slot
+    pseudo-row in row order (low word, then high word 32 slots later)."""
+    out = []
+    for r in range(9):
+        for half in range(2):
+            w = (int(masks[r]) >> (32 * half)) & 0xFFFFFFFF
+            if w:
+                out.append((w, int(first[r]) + 32 * half))
+    return out
+
+
+@pytest.mark.parametrize("case", ["sixteen_then_empty", "sixteen_then_short_rows", "seventeen", "mixed", "empty"])
+def test_flat_walk_table(case):
+    """The flattened walk's table (built by the kernels' own code) holds
+    exactly the non-empty pseudo-rows.  Exactly 16 of them followed by
+    empty ones (an empty row, or the zero high word of a row of at most 32
+    slots) used to overwrite entry 0 with an empty mask, so the walk of
+    such a particle visited no neighbour."""
+    first = np.arange(9, dtype=np.uint32) * 1000 + 7
+    masks = np.zeros(9, np.uint64)
+    if case == "sixteen_then_empty":  # rows 0-7: 33..64 slots, both words set; row 8 empty
+        masks[:8] = np.uint64((0x5 << 32) | 0x3)
+    elif case == "sixteen_then_short_rows":  # 16 entries, then a row of <= 32 slots with no neighbour
+        # 7 rows of 33..64 slots (14 entries), then two rows of <= 32 slots:
+        # entries 15 and 16, each followed by its empty high word
+        masks[:7] = np.uint64((0x80000000 << 32) | 0x1)
+        masks[7] = np.uint64(0x10)
+        masks[8] = np.uint64(0x20)
+    elif case == "seventeen":
+        masks[:8] = np.uint64((0x5 << 32) | 0x3)
+        masks[8] = np.uint64(0x1)
+    elif case == "mixed":
+        masks[[0, 2, 5, 8]] = [np.uint64(0xF0F0), np.uint64(1 << 40), np.uint64((3 << 32) | 9), np.uint64(0x2)]
+    em, eb, cnt = L.flat_table(masks, first)
+    want = _flat_reference(masks, first)
+    assert cnt == len(want)
+    k = min(cnt, L.FLAT_ENTRIES)
+    assert [(int(a), int(b)) for a, b in zip(em[:k], eb[:k])] == want[:k]
+    assert all(int(a) != 0 for a in em[:k])
+    if case.startswith("sixteen"):
+        assert cnt == 16
+    if case == "seventeen":
+        assert cnt == 17  # above the table: the kernels walk such a wave row by row

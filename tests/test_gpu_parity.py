@@ -515,7 +515,7 @@ def test_full_size_spray_c3(gpu, oracle_omp):
                     tol={"vel": 1e-4, "acc": 3e-5})
 
 
-@pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed"])
+@pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed", "cloud"])
 def test_neighbour_masks_are_bitwise_transparent(gpu, case):
     """Walking the density pass's neighbour masks, as one flattened stream,
     row by row, or flattened only in waves with rows of more than 32 slots,
@@ -528,10 +528,15 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
     dense block has stencil rows longer
     than 64 slots (those rows are scanned in every pass, and their waves walk
     row by row); the packed block (about 12 particles a cell) has rows of 33
-    to 64 slots, which the flattened walk splits into two 32-slot parts."""
+    to 64 slots, which the flattened walk splits into two 32-slot parts; the
+    random cloud (about 18 particles a cell) holds a lane whose flattened
+    table has exactly 16 entries followed by an empty pseudo-row, in a wave
+    that walks it (checked on the CPU by
+    test_oracle.py::test_cloud_scene_reaches_the_full_flat_table)."""
     cfg = {"c1": configs.dambreak_c1, "double_dam": lambda: configs.double_dam(n=4096, res=16),
            "dense": configs.dense_block,
-           "packed": lambda: configs.dense_block(24, 24, 24, spacing=configs.H / 2.3)}[case]()
+           "packed": lambda: configs.dense_block(24, 24, 24, spacing=configs.H / 2.3),
+           "cloud": configs.cloud}[case]()
     outs = []
     for scan, flat in ((True, 1), (False, 1), (False, 0), (False, 2)):
         sc = gpu.Scene(cfg.params, cfg.n)

@@ -357,3 +357,17 @@ def test_openmp_build_is_bit_identical(oracle, oracle_omp):
         outs.append((p, v, a, o["density"], o["f_pressure"]))
     for x, y in zip(*outs):
         assert np.array_equal(x, y)
+
+
+def test_cloud_scene_reaches_the_full_flat_table(oracle):
+    """The 'cloud' scene of the mask-transparency GPU test holds a lane whose
+    flattened-walk table has exactly 16 entries followed by an empty
+    pseudo-row, in a wave that walks it (the case that lost a lane's
+    neighbours before the table stored only non-empty entries)."""
+    from libfluid_amd import configs
+    from tests.helpers import flat_table_hazards
+    cfg = configs.cloud()
+    keys, _ = oracle.hash(cfg.params, cfg.pos)
+    order = oracle.sort(keys)
+    start = oracle.cell_start(cfg.params, keys)
+    assert flat_table_hazards(cfg.params, cfg.pos, order, start) >= 1
