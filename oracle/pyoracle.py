@@ -86,11 +86,15 @@ def _f64(a):
 
 
 class Oracle:
-    def __init__(self, omp: bool = False):
-        name = "liblf_oracle_omp.so" if omp else "liblf_oracle.so"
-        path = os.path.join(_HERE, "lib", name)
-        if not os.path.exists(path):
-            build()
+    def __init__(self, omp: bool = False, path: str | None = None):
+        """The built oracle (single-threaded or OpenMP), or the library at
+        `path` (a build of a modified copy, used to check that the tests pin
+        the oracle)."""
+        if path is None:
+            name = "liblf_oracle_omp.so" if omp else "liblf_oracle.so"
+            path = os.path.join(_HERE, "lib", name)
+            if not os.path.exists(path):
+                build()
         lib = C.CDLL(path)
         P = C.POINTER
         op = P(OrcParams)

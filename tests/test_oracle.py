@@ -371,3 +371,68 @@ def test_cloud_scene_reaches_the_full_flat_table(oracle):
     order = oracle.sort(keys)
     start = oracle.cell_start(cfg.params, keys)
     assert flat_table_hazards(cfg.params, cfg.pos, order, start) >= 1
+
+
+# ---- force side against closed-form vectors (tests/golden/make_kat.py) -------------
+
+def _kat_rel(got, want):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    return float(np.max(np.abs(got - want)) / np.max(np.abs(want)))
+
+
+@pytest.mark.parametrize("field", ["density", "pressure", "sound_speed", "normal", "f_pressure", "f_viscosity",
+                                   "f_tension"])
+def test_triple_against_closed_form(oracle, field):
+    """Three particles with different densities and pressures, every force
+    term on: the oracle's density, EOS, normal and force components equal
+    the 50-digit evaluation of the reference's formulas.  Distinct densities
+    make the swapped mid-term indices (Q5), the rho_i + rho_i tension
+    denominator (Q7) and the 1/rho_j of the normal observable; the pairs
+    both approach and separate (viscosity switch)."""
+    k = KAT["triple"]
+    p = make_params(**k["params"])
+    out, *_ = _step(oracle, p, k["pos"], vel=k["vel"], dt=1e-6)
+    assert _kat_rel(out[field], k["expect"][field]) < 1e-11, (out[field], k["expect"][field])
+
+
+def test_triple_pairs_cover_both_viscosity_branches():
+    """The triple's pairs include approaching and separating ones."""
+    k = KAT["triple"]
+    x, v = np.array(k["pos"]), np.array(k["vel"])
+    ux = [-np.dot(v[i] - v[j], x[j] - x[i]) for i in range(3) for j in range(3) if i != j]
+    assert min(ux) < 0.0 < max(ux)
+
+
+def test_triple_xsph_against_closed_form(oracle):
+    """XSPH (Jacobi) over the moved positions with the unmoved densities:
+    zero stiffness, forces and gravity leave v unchanged by the leapfrog and
+    move x by v dt; the final velocities equal the closed form, which
+    carries the 2 m / (rho_i + rho_j) weight."""
+    k = KAT["triple_xsph"]
+    p = make_params(**k["params"])
+    out, pos, vel, acc = _step(oracle, p, k["pos"], vel=k["vel"], dt=k["dt"], want=("vel_half",))
+    np.testing.assert_allclose(out["vel_half"], k["vel"], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(pos, k["expect"]["pos"], rtol=1e-14, atol=0)
+    assert _kat_rel(vel, k["expect"]["vel"]) < 1e-11
+    assert _kat_rel(vel - np.array(k["vel"]), np.array(k["expect"]["vel"]) - np.array(k["vel"])) < 1e-10
+
+
+def test_compressed_lattice_surface_is_pushed_outward(oracle):
+    """Sign check from physics, not from the formula: a small lattice packed
+    tighter than its rest spacing has positive pressures, and the pressure
+    force on its surface particles points away from the lattice centre (the
+    integrated acceleration too); with the lattice stretched, the pressures
+    are negative and the surface is pulled inward."""
+    h = 0.02
+    spacing = 0.4 * h
+    ax = 0.5 + spacing * (np.arange(5) - 2.0)
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    centre = pos.mean(0)
+    surf = np.flatnonzero(np.max(np.abs(pos - centre), axis=1) > 1.5 * spacing)
+    # heavy particles: every density above rest (compressed); light: below
+    for mass, sign in ((4e-3, 1.0), (2e-4, -1.0)):
+        p = lone_params(mass=mass)
+        out, *_ = _step(oracle, p, pos, want=("density", "pressure", "f_pressure"))
+        assert np.all(np.sign(out["pressure"]) == sign), (mass, out["density"].min(), out["density"].max())
+        radial = np.einsum("ij,ij->i", out["f_pressure"][surf], pos[surf] - centre)
+        assert np.all(np.sign(radial) == sign), f"mass {mass}: surface force not {'out' if sign > 0 else 'in'}ward"
