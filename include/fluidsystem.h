@@ -54,7 +54,10 @@ public:
                        const uint &_res = 32);
 
     /// Copy positions, density colour and velocity to the host, in the current
-    /// cell-sorted order (fluidsystem.cu:154-173)
+    /// cell-sorted order (fluidsystem.cu:154-173).  The reference copies into
+    /// vectors the caller sized to the particle count (main.cpp:119-121);
+    /// here a vector shorter than that is grown to it first (longer ones keep
+    /// their size and tail).
     void exportToData(std::vector<float3> &points, std::vector<float3> &colour, std::vector<float3> &velocity);
 
     /// Retrieve the current particle scale

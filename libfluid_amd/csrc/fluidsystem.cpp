@@ -94,6 +94,11 @@ void FluidSystem::createFluidBlock(const uint &firstPtIdx, const uint &numPoints
         return;
     }
     if (numPoints == 0) return;
+    // The block's mass follows the current rest density (fluidsystem.cu:105).
+    if (m_dirty) {
+        if (!check(lf_set_params(m_scene, &m_params), "createFluidBlock")) return;
+        m_dirty = false;
+    }
     const float mn[3] = {min_x, min_y, min_z}, mx[3] = {max_x, max_y, max_z};
     uint32_t written = 0;
     float mass = 0.0f, psize = 0.0f;

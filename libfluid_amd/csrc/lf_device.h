@@ -13,11 +13,9 @@ constexpr uint32_t kNoScene = 0xFFFFFFFFu;    // chunk owned by no scene
 constexpr uint32_t kMaxParticles = 1u << 28;  // pool slots: 16-byte records addressed with 32-bit offsets
 // Stable placement by (cell, id): a bucket of at most kRankDirectMax
 // particles is ranked by id directly (each member counts the smaller ids,
-// O(size^2) per bucket); a larger one is sorted by id as a segment
-// (k_sort_segments: LSD radix sort, O(size) per pass).
+// O(size^2) per bucket); a larger one is sorted by id as a segment (LSD
+// radix sort, O(size) per pass, k_sort_segments).
 constexpr uint32_t kRankDirectMax = 512;
-constexpr int kSegThreads = 1024;  // workgroup of k_sort_segments: one segment at a time
-constexpr int kSegBlocks = 128;    // its workgroups (segments are taken in a grid stride)
 constexpr int kScanPerThread = 4;             // cell-count scan: entries per thread (one 16-byte load)
 constexpr int kScanTile = kBlock * kScanPerThread;
 constexpr uint32_t kScanDirectTiles = 2048;    // above this many tiles the tile prefixes get their own pass

@@ -42,7 +42,7 @@ void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t 
 void launch_scan(const Launch &L, uint32_t *counts, uint32_t m, uint32_t *bsum, uint32_t *start);
 // Provisional slots by atomic rank; then every bucket of more than
 // kRankDirectMax particles is sorted by id in place (its segment listed in
-// segs, nsegs of them; tmp: one uint2 per pool slot, scratch).
+// segs, nsegs[0] of them; tmp: one uint2 per pool slot, scratch).
 void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const uint32_t *keys, const uint32_t *ranks,
                     const uint32_t *start, uint32_t *slot_src, uint32_t *slot_id, uint4 *segs, uint32_t *nsegs,
                     uint2 *tmp);
@@ -52,9 +52,12 @@ void launch_scatter(const Launch &L, const PoolView &pv, const float4 *P, const 
 void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src, const uint32_t *slot_id,
                    const uint32_t *keys, const uint32_t *start, const float4 *P, const float4 *V, float4 *Ps,
                    float4 *Vs, uint32_t *order, float *soa, uint32_t *moves);
+// Writes one set of neighbour masks, MS: the slots within the substep's
+// skin radius (at least h), which the normals and forces passes walk
+// (re-testing r <= h) and XSPH walks after the move.
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
-                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
-                    uint32_t *flags, const uint32_t *moves);
+                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks MS, uint32_t *flags,
+                    const uint32_t *moves);
 // The neighbour passes take `flat`: 1 walks the masks as one flattened
 // stream of set bits (where the rows fit) rather than row by row (0), 2 does
 // so only in waves with a row of more than 32 slots; results are the same

@@ -187,10 +187,120 @@ __device__ __forceinline__ uint32_t slot_shift(const Consts &c, const uint32_t *
     return c.first - start[c.cell_base];
 }
 
+// 3b. segments (buckets of more than kRankDirectMax particles) sorted by id
+// in place: (slot_id, slot_src) of slots [lo, lo + size), by one workgroup of
+// NT threads, segment after segment (k_sort_segments, after k_scatter): an LSD radix sort by 8 bits of the id
+// per pass, through tmp (same slots) and back.  A pass counts the digits,
+// then places the segment tile by tile (NT elements, in slot order): a
+// member's new place is the digit's running base, plus that digit's count
+// in the tile's earlier waves, plus its rank among the earlier lanes of its
+// wave with the same digit (lanes matched by eight ballots), so every pass
+// is stable.  Ids are unique within a scene.
+__device__ __forceinline__ uint32_t seg_digit(uint32_t id, uint32_t shift) { return (id >> shift) & 0xFFu; }
+
+template <int NT>
+struct SegSortLds {
+    uint32_t base[256];               // running start of each digit in the destination
+    uint32_t wcnt[NT / 64][256];      // per wave: digit counts, then their prefix over waves
+    uint32_t scan_tot[4];
+};
+
+template <int NT>
+__device__ void sort_segments(SegSortLds<NT> &sh, const uint4 *__restrict__ segs, uint32_t nseg,
+                              uint32_t *__restrict__ slot_id, uint32_t *__restrict__ slot_src,
+                              uint2 *__restrict__ tmp) {
+    static_assert(NT >= 256 && NT % 64 == 0, "the digit scan needs four waves");
+    constexpr int kWaves = NT / 64;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t sg = 0; sg < nseg; ++sg) {
+        const uint4 g = segs[sg];
+        const uint32_t lo = g.x, n = g.y, passes = (g.z + 7u) / 8u;
+        for (uint32_t p = 0; p < passes; ++p) {
+            const bool from_tmp = (p & 1u) != 0u;
+            const uint32_t shift = 8u * p;
+            if (t < 256u) sh.base[t] = 0u;
+            __syncthreads();
+            for (uint32_t k = t; k < n; k += NT) {
+                const uint32_t id = from_tmp ? tmp[lo + k].x : slot_id[lo + k];
+                atomicAdd(&sh.base[seg_digit(id, shift)], 1u);
+            }
+            __syncthreads();
+            // exclusive scan of the 256 digit counts (waves 0-3)
+            uint32_t v = 0u, incl = 0u;
+            if (t < 256u) {
+                v = sh.base[t];
+                incl = v;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t u = __shfl_up(incl, o, 64);
+                    if ((int)lane >= o) incl += u;
+                }
+                if (lane == 63u) sh.scan_tot[w] = incl;
+            }
+            __syncthreads();
+            if (t < 256u) {
+                uint32_t off = 0u;
+                for (uint32_t q = 0; q < w; ++q) off += sh.scan_tot[q];
+                sh.base[t] = off + incl - v;
+            }
+            __syncthreads();
+            for (uint32_t tile = 0; tile < n; tile += NT) {
+                for (uint32_t q = t; q < kWaves * 256u; q += NT) (&sh.wcnt[0][0])[q] = 0u;
+                __syncthreads();
+                const uint32_t k = tile + t;
+                const bool valid = k < n;
+                uint2 e = make_uint2(0u, 0u);
+                if (valid) e = from_tmp ? tmp[lo + k] : make_uint2(slot_id[lo + k], slot_src[lo + k]);
+                const uint32_t d = seg_digit(e.x, shift);
+                uint64_t peers = __ballot(valid);
+#pragma unroll
+                for (uint32_t bit = 0; bit < 8u; ++bit) {
+                    const bool on = ((d >> bit) & 1u) != 0u;
+                    const uint64_t m = __ballot(on);
+                    peers &= on ? m : ~m;
+                }
+                const uint32_t rank = (uint32_t)__popcll(peers & below);
+                if (valid && rank == 0u) sh.wcnt[w][d] = (uint32_t)__popcll(peers);
+                __syncthreads();
+                uint32_t tile_total = 0u;
+                if (t < 256u) {  // prefix of digit t's counts over the waves
+                    for (uint32_t q = 0; q < (uint32_t)kWaves; ++q) {
+                        const uint32_t c = sh.wcnt[q][t];
+                        sh.wcnt[q][t] = tile_total;
+                        tile_total += c;
+                    }
+                }
+                __syncthreads();
+                if (valid) {
+                    const uint32_t dst = lo + sh.base[d] + sh.wcnt[w][d] + rank;
+                    if (from_tmp) {
+                        slot_id[dst] = e.x;
+                        slot_src[dst] = e.y;
+                    } else {
+                        tmp[dst] = e;
+                    }
+                }
+                __syncthreads();
+                if (t < 256u) sh.base[t] += tile_total;
+                __syncthreads();
+            }
+        }
+        if (passes & 1u) {  // the last pass wrote tmp
+            for (uint32_t k = t; k < n; k += NT) {
+                const uint2 e = tmp[lo + k];
+                slot_id[lo + k] = e.x;
+                slot_src[lo + k] = e.y;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // 3. provisional scatter: slot = bucket start + atomic rank.  The particle
 // with rank 0 of a bucket of more than kRankDirectMax particles lists the
-// bucket's slot range as a segment for k_sort_segments: (first slot,
-// size, bits of the scene's largest id).
+// bucket's slot range as a segment for k_sort_segments: (first slot, size,
+// bits of the scene's largest id).
 __global__ __launch_bounds__(kBlock) void k_scatter(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                     const float4 *__restrict__ P, const uint32_t *__restrict__ keys,
                                                     const uint32_t *__restrict__ ranks,
@@ -216,111 +326,21 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const Consts *__restrict__ c
     }
 }
 
-// 3b. segments (buckets of more than kRankDirectMax particles) sorted by id
-// in place: (slot_id, slot_src) of slots [lo, lo + size).  One workgroup per
-// segment (grid stride over the list k_scatter wrote), LSD radix sort by 8
-// bits of the id per pass, through tmp (same slots) and back.  A pass
-// counts the digits, then places the segment tile by tile (kSegThreads
-// elements, in slot order): a member's new place is the digit's running
-// base, plus that digit's count in the tile's earlier waves, plus its rank
-// among the earlier lanes of its wave with the same digit (lanes matched by
-// eight ballots), so every pass is stable.  Ids are unique within a scene.
-__device__ __forceinline__ uint32_t seg_digit(uint32_t id, uint32_t shift) { return (id >> shift) & 0xFFu; }
-
+// 3c. the listed segments, one workgroup each (grid stride over the list;
+// a workgroup with no segment exits at once).  (A ticket taken by every
+// workgroup of the scatter pass, to let its last workgroup sort, needs a
+// device-scope fence per wave, which costs far more than this launch.)
+constexpr int kSegThreads = 1024;
+constexpr int kSegBlocks = 32;
 __global__ __launch_bounds__(kSegThreads) void k_sort_segments(const uint4 *__restrict__ segs,
                                                                const uint32_t *__restrict__ nsegs,
                                                                uint32_t *__restrict__ slot_id,
                                                                uint32_t *__restrict__ slot_src,
                                                                uint2 *__restrict__ tmp) {
-    constexpr int kWaves = kSegThreads / 64;
-    __shared__ uint32_t base[256];           // running start of each digit in the destination
-    __shared__ uint32_t wcnt[kWaves][256];   // per wave: digit counts, then their prefix over waves
-    __shared__ uint32_t scan_tot[4];
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint64_t below = (1ull << lane) - 1ull;
+    __shared__ SegSortLds<kSegThreads> sh;
     const uint32_t nseg = *nsegs;
-    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x) {
-        const uint4 g = segs[sg];
-        const uint32_t lo = g.x, n = g.y, passes = (g.z + 7u) / 8u;
-        for (uint32_t p = 0; p < passes; ++p) {
-            const bool from_tmp = (p & 1u) != 0u;
-            const uint32_t shift = 8u * p;
-            if (t < 256u) base[t] = 0u;
-            __syncthreads();
-            for (uint32_t k = t; k < n; k += kSegThreads) {
-                const uint32_t id = from_tmp ? tmp[lo + k].x : slot_id[lo + k];
-                atomicAdd(&base[seg_digit(id, shift)], 1u);
-            }
-            __syncthreads();
-            // exclusive scan of the 256 digit counts (waves 0-3)
-            uint32_t v = 0u, incl = 0u;
-            if (t < 256u) {
-                v = base[t];
-                incl = v;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const uint32_t u = __shfl_up(incl, o, 64);
-                    if ((int)lane >= o) incl += u;
-                }
-                if (lane == 63u) scan_tot[w] = incl;
-            }
-            __syncthreads();
-            if (t < 256u) {
-                uint32_t off = 0u;
-                for (uint32_t q = 0; q < w; ++q) off += scan_tot[q];
-                base[t] = off + incl - v;
-            }
-            __syncthreads();
-            for (uint32_t tile = 0; tile < n; tile += kSegThreads) {
-                for (uint32_t q = t; q < kWaves * 256u; q += kSegThreads) (&wcnt[0][0])[q] = 0u;
-                __syncthreads();
-                const uint32_t k = tile + t;
-                const bool valid = k < n;
-                uint2 e = make_uint2(0u, 0u);
-                if (valid) e = from_tmp ? tmp[lo + k] : make_uint2(slot_id[lo + k], slot_src[lo + k]);
-                const uint32_t d = seg_digit(e.x, shift);
-                uint64_t peers = __ballot(valid);
-#pragma unroll
-                for (uint32_t bit = 0; bit < 8u; ++bit) {
-                    const bool on = ((d >> bit) & 1u) != 0u;
-                    const uint64_t m = __ballot(on);
-                    peers &= on ? m : ~m;
-                }
-                const uint32_t rank = (uint32_t)__popcll(peers & below);
-                if (valid && rank == 0u) wcnt[w][d] = (uint32_t)__popcll(peers);
-                __syncthreads();
-                uint32_t tile_total = 0u;
-                if (t < 256u) {  // prefix of digit t's counts over the waves
-                    for (uint32_t q = 0; q < (uint32_t)kWaves; ++q) {
-                        const uint32_t c = wcnt[q][t];
-                        wcnt[q][t] = tile_total;
-                        tile_total += c;
-                    }
-                }
-                __syncthreads();
-                if (valid) {
-                    const uint32_t dst = lo + base[d] + wcnt[w][d] + rank;
-                    if (from_tmp) {
-                        slot_id[dst] = e.x;
-                        slot_src[dst] = e.y;
-                    } else {
-                        tmp[dst] = e;
-                    }
-                }
-                __syncthreads();
-                if (t < 256u) base[t] += tile_total;
-                __syncthreads();
-            }
-        }
-        if (passes & 1u) {  // the last pass wrote tmp
-            for (uint32_t k = t; k < n; k += kSegThreads) {
-                const uint2 e = tmp[lo + k];
-                slot_id[lo + k] = e.x;
-                slot_src[lo + k] = e.y;
-            }
-        }
-        __syncthreads();
-    }
+    for (uint32_t sg = blockIdx.x; sg < nseg; sg += gridDim.x)
+        sort_segments<kSegThreads>(sh, segs + sg, 1u, slot_id, slot_src, tmp);
 }
 
 // 4. stable placement + payload gather (sort_by_key of Points, Velocity,
@@ -569,7 +589,7 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
                                                 uint32_t nslots, const float4 *__restrict__ Ps,
                                                 const float *__restrict__ soa, const uint32_t *__restrict__ start,
                                                 float4 *__restrict__ D,
-                                                float4 *__restrict__ Q, float *__restrict__ Pr, RowMasks M,
+                                                float4 *__restrict__ Q, float *__restrict__ Pr,
                                                 RowMasks MS, uint32_t *__restrict__ flags,
                                                 const uint32_t *__restrict__ moves, uint8_t *__restrict__ near) {
     uint32_t slot0;
@@ -609,30 +629,32 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         const uint32_t bb = b[r], len = e[r] - b[r];
         const uint32_t off = bb * (uint32_t)sizeof(float);
         const uint32_t kself = i - bb;  // < len only in the centre row
-        const uint32_t lw = wave_max<7>(len < 64u ? len : 64u);
+        // Slots scanned: up to the wave's longest row (at most 64), rounded up
+        // to a multiple of 4; slots past a lane's row add nothing.
+        const uint32_t lw = (wave_max<7>(len < 64u ? len : 64u) + 3u) & ~3u;
         const uint32_t lw0 = lw < 32u ? lw : 32u;
-        uint32_t m0 = 0u, s0 = 0u, m1 = 0u, s1 = 0u;
-        auto test = [&](uint32_t k, float qx, float qy, float qz, uint32_t &m, uint32_t &sm) {
+        uint32_t s0 = 0u, s1 = 0u;
+        auto test = [&](uint32_t k, float qx, float qy, float qz, uint32_t &sm) {
             const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
             uint64_t live = lanes(k < len);
             if (r == 4) live &= ~lanes(k == kself);
             const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
             const float t = keep(c.h2 - rr, in);
             S += t * t * t;
-            m = shift_in(m, in);
             sm = shift_in(sm, ins);
         };
-        // Slots k0 .. min(k0 + 4, kend) - 1 (kend wave-uniform).
-        auto four = [&](uint32_t k0, uint32_t kend, uint32_t &m, uint32_t &sm) {
+        // Slots k0 .. k0 + 3 (every lane past its row's end for all four
+        // when k0 >= its length).
+        auto four = [&](uint32_t k0, uint32_t &sm) {
             const uint32_t so = k0 * (uint32_t)sizeof(float);
             const float4 X = load4(px, off, so), Y = load4(py, off, so), Z = load4(pz, off, so);
-            test(k0, X.x, Y.x, Z.x, m, sm);
-            if (k0 + 1u < kend) test(k0 + 1u, X.y, Y.y, Z.y, m, sm);
-            if (k0 + 2u < kend) test(k0 + 2u, X.z, Y.z, Z.z, m, sm);
-            if (k0 + 3u < kend) test(k0 + 3u, X.w, Y.w, Z.w, m, sm);
+            test(k0, X.x, Y.x, Z.x, sm);
+            test(k0 + 1u, X.y, Y.y, Z.y, sm);
+            test(k0 + 2u, X.z, Y.z, Z.z, sm);
+            test(k0 + 3u, X.w, Y.w, Z.w, sm);
         };
-        for (uint32_t k = 0; k < lw0; k += 4u) four(k, lw0, m0, s0);
-        for (uint32_t k = 32u; k < lw; k += 4u) four(k, lw, m1, s1);
+        for (uint32_t k = 0; k < lw0; k += 4u) four(k, s0);
+        for (uint32_t k = 32u; k < lw; k += 4u) four(k, s1);
         for (uint32_t k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
             const float rr = dist2(pi, at32(Ps, bb + k));
             const bool in = rr <= c.h2 && (r != 4 || k != kself);
@@ -640,15 +662,12 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             S += t * t * t;
         }
         const size_t at = (size_t)r * nslots + i;
-        const uint32_t sh0 = 32u - lw0;  // lw0 == 0 leaves m0 = s0 = 0
-        m0 = lw0 ? __brev(m0) >> sh0 : 0u;
+        const uint32_t sh0 = 32u - lw0;  // lw0 == 0 leaves s0 = 0
         s0 = lw0 ? __brev(s0) >> sh0 : 0u;
         if (len <= 32u) {
-            M.w32[at] = m0;
             MS.w32[at] = s0;
         } else if (len <= 64u) {
             const uint32_t sh1 = 64u - lw;  // lw - 32 slots scanned into the high word (lw >= len > 32)
-            M.w64[at] = (uint64_t)(__brev(m1) >> sh1) << 32 | m0;
             MS.w64[at] = (uint64_t)(__brev(s1) >> sh1) << 32 | s0;
         }
     }
@@ -661,11 +680,14 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             const uint32_t len = e[r] - b[r];
             const size_t at = (size_t)r * nslots + i;
             if (len <= 64u) {
-                uint64_t m = len <= 32u ? (uint64_t)M.w32[at] : M.w64[at];
+                // the skin mask: a superset of the pairs within h
+                uint64_t m = len <= 32u ? (uint64_t)MS.w32[at] : MS.w64[at];
                 while (m) {
                     const uint32_t j = b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u;
                     m &= m - 1ull;
-                    const double t = fmax(c.hh_d - dist2_d(pi, at32(Ps, j)), 0.0);
+                    const float4 q = at32(Ps, j);
+                    if (!(dist2(pi, q) <= c.h2)) continue;
+                    const double t = fmax(c.hh_d - dist2_d(pi, q), 0.0);
                     Sd += t * t * t;
                 }
             } else {
@@ -1376,10 +1398,10 @@ void launch_gather(const Launch &L, const PoolView &pv, const uint32_t *slot_src
 }
 
 void launch_density(const Launch &L, const PoolView &pv, int bs, const float4 *Ps, const float *soa,
-                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks M, RowMasks MS,
-                    uint32_t *flags, const uint32_t *moves) {
+                    const uint32_t *start, float4 *D, float4 *Q, float *Pr, RowMasks MS, uint32_t *flags,
+                    const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_density<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr, M,
+        LF_GO(L, k_density<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, soa, start, D, Q, Pr,
               MS, flags, moves, pv.near);
     });
 }

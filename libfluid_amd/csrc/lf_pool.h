@@ -165,7 +165,7 @@ class Pool {
     float4 *D_ = nullptr, *N_ = nullptr, *F_ = nullptr, *Q_ = nullptr, *X_ = nullptr;
     float *Pr_ = nullptr;
     float *soa_ = nullptr;  // sorted positions as three planes of nslots floats (x, y, z)
-    RowMasks M_{nullptr, nullptr}, MS_{nullptr, nullptr};
+    RowMasks MS_{nullptr, nullptr};  // neighbour masks at the skin radius (density pass)
     uint32_t *keys_ = nullptr, *ranks_ = nullptr, *slot_src_ = nullptr, *slot_id_ = nullptr;
     uint32_t *counts_ = nullptr, *start_ = nullptr, *bsum_ = nullptr;
     uint4 *segs_ = nullptr;      // buckets of more than kRankDirectMax particles: (first slot, size, id bits)

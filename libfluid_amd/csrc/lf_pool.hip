@@ -149,11 +149,8 @@ void Pool::release() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     stop_exporter();
     drop_graph();
-    RowMasks *masks[] = {&M_, &MS_};
-    for (RowMasks *m : masks) {
-        dev_free(m->w32);
-        dev_free(m->w64);
-    }
+    dev_free(MS_.w32);
+    dev_free(MS_.w64);
     float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &F_, &Q_, &X_};
     for (float4 **b : f4) dev_free(*b);
     dev_free(Pr_);
@@ -211,9 +208,7 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
         if (!rc) rc = dalloc(b, n);
     if (!rc) rc = dalloc(&Pr_, n);
     if (!rc) rc = dalloc(&soa_, 3 * n);
-    if (!rc) rc = dalloc(&M_.w32, 9 * n + 2);  // +2: an 8-byte read at the last entry stays inside
-    if (!rc) rc = dalloc(&M_.w64, 9 * n);
-    if (!rc) rc = dalloc(&MS_.w32, 9 * n + 2);
+    if (!rc) rc = dalloc(&MS_.w32, 9 * n + 2);  // +2: an 8-byte read at the last entry stays inside
     if (!rc) rc = dalloc(&MS_.w64, 9 * n);
     uint32_t **u32[] = {&keys_, &ranks_, &slot_src_, &slot_id_};
     for (uint32_t **b : u32)
@@ -559,15 +554,15 @@ void Pool::enqueue_substep() {
     const PoolView pv = view();
     enqueue_sort();
     timed(LF_K_DENSITY, [&](const Launch &L) {
-        launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, M_, MS_, flags_, moves_);
+        launch_density(L, pv, tune_.density, P1_, soa_, start_, D_, Q_, Pr_, MS_, flags_, moves_);
     });
     if (any_separate_normals())
         timed(LF_K_NORMALS, [&](const Launch &L) {
-            launch_normals(L, pv, tune_.normals, tune_.flat_normals, false, Q_, start_, M_, flags_, N_);
+            launch_normals(L, pv, tune_.normals, tune_.flat_normals, false, Q_, start_, MS_, flags_, N_);
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
         launch_forces(L, pv, tune_.forces, tune_.flat_forces, P1_, Q_, Pr_, V1_, A0_, L0_, ranks_, D_, N_, start_,
-                      M_, P0_, V2_, A1_, L1_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
+                      MS_, P0_, V2_, A1_, L1_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
         launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, X_, V2_, L1_, start_, MS_, flags_, V0_);
@@ -804,7 +799,7 @@ int Pool::get_field(uint32_t id, int field, void *dst, size_t bytes) {
                 return fail(LF_ERR_STATE, "normals: no substep data for this scene (none since it was added, "
                                           "loaded, moved, re-parametrised or paused, or autotuning ran since)");
             launch_normals(Launch{stream_, nullptr, nullptr}, view(), tune_.normals, tune_.flat_normals, true,
-                           Q_, start_, M_, flags_, N_);
+                           Q_, start_, MS_, flags_, N_);
             if (int rc = check_launch("normals")) return rc;
             LF_HIP(hipStreamSynchronize(stream_));
             src = N_ + f;

@@ -301,13 +301,15 @@ int lf_batch_footprint(uint32_t max_particles, uint32_t max_cells, uint32_t max_
     if (!bytes) return fail(LF_ERR_INVALID, "null argument");
     const uint64_t slots = ((uint64_t)max_particles + lf::kChunk - 1) / lf::kChunk * lf::kChunk;
     // 13 float4 arrays, pressure, sorted positions as three float planes,
-    // 9 x (32 + 64)-bit masks at two radii, four uint32 sort arrays per slot;
+    // 9 x (32 + 64)-bit masks, four uint32 sort arrays per slot;
     // counts and starts per bucket; per scene its constants, flag word and
     // move table (kMoveSlots words one cache line apart).
-    const uint64_t per_slot = 13 * 16 + 4 + 3 * 4 + 9 * (4 + 8) * 2 + 4 * 4;
+    const uint64_t per_slot = 13 * 16 + 4 + 3 * 4 + 9 * (4 + 8) + 4 * 4;
     const uint64_t per_scene = sizeof(lf::Consts) + 4 * lf::kSceneWords + 4 * lf::kMoveSlots * lf::kMoveStride;
+    // plus the chunk map, near-isolated marks, and the sort's segment list
+    // (one 16-byte entry per kRankDirectMax + 1 slots) and its count
     *bytes = slots * per_slot + (uint64_t)max_cells * 8 + (uint64_t)max_scenes * per_scene + slots / lf::kChunk * 4 +
-             slots / lf::kNearBlock;
+             slots / lf::kNearBlock + (slots / (lf::kRankDirectMax + 1) + 1) * 16 + 4;
     return LF_OK;
 }
 

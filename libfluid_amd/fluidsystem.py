@@ -12,7 +12,7 @@ import sys
 import numpy as np
 
 from . import _lib as L
-from .scene import LfParams, Scene, default_params, fluid_block
+from .scene import LfParams, Scene, default_params
 
 DEFAULT_TIMESTEP = 0.01
 
@@ -53,12 +53,19 @@ class FluidSystem:
             print(f"FluidSystem::createFluidBlock() - attempting to make {numPoints} points but you only "
                   f"have enough space for {self._n - firstPtIdx}\n!", file=sys.stderr)
             return
-        pts, mass, psize = fluid_block(numPoints, (min_x, min_y, min_z), (max_x, max_y, max_z),
-                                       self._params.rest_density)
+        if numPoints == 0:
+            return
+        if self._dirty:  # the block's mass follows the current rest density (fluidsystem.cu:105)
+            self._scene.set_params(self._params)
+            self._dirty = False
+        # Generated on the device, as the C++ class does (the reference
+        # writes the lattice point by point from the host,
+        # fluidsystem.cu:113-126); the block sets the mass and particle size
+        # (fluidsystem.cu:105,108).
+        _, mass, psize = self._scene.setup_fluid_block(numPoints, (min_x, min_y, min_z), (max_x, max_y, max_z),
+                                                       first=firstPtIdx)
         self.setMass(mass)
         self.setParticleSize(psize)
-        if pts.shape[0]:
-            self._scene.upload(pts, first=firstPtIdx)
 
     # -- public surface ------------------------------------------------------
     def setup(self, numPoints: int, res: int) -> None:
@@ -70,6 +77,8 @@ class FluidSystem:
         self.createFluidBlock(0, numPoints, 0.1, 0.1, 0.1, 0.9, 0.9, 0.9)
 
     def setupFromData(self, points, velocity, res: int = 32) -> None:
+        """FluidSystem::setupFromData (fluidsystem.cu:179-189): like the
+        reference, mass and particle size keep init's defaults of 1."""
         points = np.asarray(points, np.float32).reshape(-1, 3)
         self.init(points.shape[0], res)
         self._scene.upload(points, np.asarray(velocity, np.float32).reshape(-1, 3))

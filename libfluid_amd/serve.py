@@ -130,17 +130,21 @@ def served_jobs(count: int, rate: float, seed: int = 5, n_lo: int = 4096, n_hi: 
     return jobs
 
 
-def drive(server: Server, jobs: list, keep_results: bool = False, log=None) -> dict:
+def drive(server: Server, jobs: list, keep_results: bool = False, log=None, sample_s: float = 0.05) -> dict:
     """Feeds `jobs` to the server at their arrival times (wall clock from the
     call) and ticks until every job finished.  Returns wall time, per-job
-    latencies and, with keep_results, every job's final state by index."""
+    latencies, a trace of (seconds, particle-steps so far, resident
+    particles) sampled every sample_s, and, with keep_results, every job's
+    final state by index."""
     order = sorted(range(len(jobs)), key=lambda k: jobs[k].arrival)
     ids = {}
     results = {}
     lat = {}
+    trace = []
     nxt = 0
     t0 = time.perf_counter()
     last_log = t0
+    last_sample = t0
     while len(lat) < len(jobs):
         now = time.perf_counter() - t0
         while nxt < len(order) and jobs[order[nxt]].arrival <= now:
@@ -157,10 +161,32 @@ def drive(server: Server, jobs: list, keep_results: bool = False, log=None) -> d
                 lat[ids[j]] = r.pop("latency_ms")
                 if keep_results:
                     results[ids[j]] = r
-        if log and time.perf_counter() - last_log > 5.0:
-            last_log = time.perf_counter()
+        now = time.perf_counter()
+        if now - last_sample >= sample_s:
+            last_sample = now
+            st = server.stats()
+            trace.append((now - t0, st["particle_steps"], st["resident_particles"]))
+        if log and now - last_log > 5.0:
+            last_log = now
             st = server.stats()
             log(f"[serve] t={last_log - t0:.1f}s submitted {nxt}/{len(jobs)} finished {len(lat)} "
                 f"running {st['running']} queued {st['queued']} resident {st['resident_particles']}")
     wall = time.perf_counter() - t0
-    return {"wall_s": wall, "latency_ms": lat, "results": results}
+    st = server.stats()
+    trace.append((wall, st["particle_steps"], st["resident_particles"]))
+    return {"wall_s": wall, "latency_ms": lat, "results": results, "trace": trace,
+            "last_arrival_s": max(j.arrival for j in jobs) if jobs else 0.0}
+
+
+def steady_window(run: dict, slot_cap: int, jobs: list, skip: float = 0.125) -> dict:
+    """Throughput and occupancy of a driven stream between the arrival of
+    its first `skip` fraction of jobs (ramp-up) and its last arrival (after
+    which the server only drains)."""
+    arr = sorted(j.arrival for j in jobs)
+    ta, tb = arr[int(skip * len(arr))], arr[-1]
+    tr = [x for x in run["trace"] if ta <= x[0] <= tb]
+    if len(tr) < 2:
+        return {"t0_s": ta, "t1_s": tb, "value": None, "mean_occupancy": None}
+    dt = tr[-1][0] - tr[0][0]
+    return {"t0_s": tr[0][0], "t1_s": tr[-1][0], "value": (tr[-1][1] - tr[0][1]) / dt,
+            "mean_occupancy": float(np.mean([x[2] for x in tr])) / slot_cap}

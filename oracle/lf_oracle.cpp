@@ -516,6 +516,11 @@ int orc_step(const orc_params *p, double dt, uint32_t n, double *pos, double *ve
     // table of this substep and the integrated positions; Jacobi update.
     if (pin)
         for (size_t k = 0; k < 3 * (size_t)n; ++k) pf[k] = (float)pos[k];
+    std::vector<double> px;  // integrated positions as FP32 storage holds them (ORC_XSPH_F32_POS)
+    if (flags & ORC_XSPH_F32_POS) {
+        px.resize(3 * (size_t)n);
+        for (size_t k = 0; k < 3 * (size_t)n; ++k) px[k] = (double)(float)pos[k];
+    }
     std::vector<double> vh(vel, vel + 3 * (size_t)n);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t ii = 0; ii < (int64_t)n; ++ii) {
@@ -525,6 +530,7 @@ int orc_step(const orc_params *p, double dt, uint32_t n, double *pos, double *ve
         for_stencil(d, ct, keys[i], [&](uint32_t j) {
             double rr;
             if (j != i && in_range(i, j, false, rr) && (rho[i] + rho[j]) > d.eps) {
+                if (!px.empty()) rr = rr_f64(&px[3 * i], &px[3 * j]);
                 double c = poly6(d, std::sqrt(rr)) * ((2.0 * d.m) / (rho[i] + rho[j]));
                 for (int a = 0; a < 3; ++a) s[a] += c * (vh[3 * j + a] - vh[3 * i + a]);
             }
@@ -578,6 +584,10 @@ void orc_colour(const orc_params *p, double density, double *rgb) {
         rgb[1] = 2.0 - ratio;
         rgb[2] = 0.0;
     }
+}
+
+void orc_colours(const orc_params *p, uint32_t n, const double *density, double *rgb) {
+    for (uint32_t i = 0; i < n; ++i) orc_colour(p, density[i], rgb + 3 * (size_t)i);
 }
 
 }  // extern "C"

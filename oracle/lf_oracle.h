@@ -55,6 +55,9 @@ typedef struct orc_params {
 
 /* Step flags. */
 #define ORC_PIN_F32 1u   /* evaluate cell keys and the pair predicate in FP32 on fl32(x) */
+#define ORC_XSPH_F32_POS 2u  /* XSPH weights from the integrated positions rounded to FP32, as an
+                                    implementation storing positions in FP32 (the reference's float3
+                                    arrays) sees them: the FP32-storage floor of the XSPH velocity */
 
 /* Optional per-step outputs.  Every pointer may be NULL.  Per-particle arrays
  * are indexed by particle id; `order` is slot -> id in the new cell order;
@@ -110,6 +113,8 @@ double orc_boundary_density(const orc_params *p, double d);
 double orc_sdf(const orc_params *p, const double *x);
 void orc_sdf_normal(const orc_params *p, const double *x, double *n);
 void orc_colour(const orc_params *p, double density, double *rgb);
+/* orc_colour of n densities into rgb[3n]. */
+void orc_colours(const orc_params *p, uint32_t n, const double *density, double *rgb);
 
 #ifdef __cplusplus
 }

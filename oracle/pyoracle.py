@@ -13,6 +13,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 PIN_F32 = 1
+XSPH_F32_POS = 2  # XSPH weights from FP32-rounded integrated positions (storage floor)
 
 
 class OrcParams(C.Structure):
@@ -118,6 +119,7 @@ class Oracle:
             ("orc_sdf", C.c_double, [op, P(C.c_double)]),
             ("orc_sdf_normal", None, [op, P(C.c_double), P(C.c_double)]),
             ("orc_colour", None, [op, C.c_double, P(C.c_double)]),
+            ("orc_colours", None, [op, C.c_uint32, P(C.c_double), P(C.c_double)]),
         ]:
             f = getattr(lib, fn)
             f.restype = res
@@ -220,4 +222,11 @@ class Oracle:
     def colour(self, params, density):
         rgb = np.empty(3, np.float64)
         self.lib.orc_colour(C.byref(to_orc(params)), float(density), _f64(rgb))
+        return rgb
+
+    def colours(self, params, density):
+        """DensityColourOperator of every density: (n, 3)."""
+        d = np.ascontiguousarray(density, np.float64)
+        rgb = np.empty((d.size, 3), np.float64)
+        self.lib.orc_colours(C.byref(to_orc(params)), d.size, _f64(d), _f64(rgb))
         return rgb

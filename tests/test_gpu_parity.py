@@ -25,7 +25,7 @@ import pytest
 
 from libfluid_amd import _lib as L
 from libfluid_amd import configs
-from oracle.pyoracle import PIN_F32
+from oracle.pyoracle import PIN_F32, XSPH_F32_POS
 from tests.helpers import by_id, normwise, rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,7 @@ def gpu_substep(lf, cfg, pos, vel, acc):
         density=by_id(D[:, 0], o), pressure=by_id(D[:, 1], o), sound_speed=by_id(D[:, 2], o),
         normal=by_id(N[:, :3], o), f_pressure=by_id(F[:, 0, :3], o), f_viscosity=by_id(F[:, 1, :3], o),
         f_tension=by_id(F[:, 2, :3], o), f_adhesion=by_id(F[:, 3, :3], o), vel_half=by_id(Vh[:, :3], o),
-        pos=st["pos"], vel=st["vel"], acc=st["acc"],
+        pos=st["pos"], vel=st["vel"], acc=st["acc"], colour=st["colour"],
     )
     assert np.array_equal(st["ids"], np.arange(pos.shape[0], dtype=np.uint32))
     return r
@@ -67,7 +67,7 @@ def gpu_substep(lf, cfg, pos, vel, acc):
 def oracle_substep(orc, cfg, pos, vel, acc, flags=PIN_F32):
     p64, v64, a64 = (np.array(x, dtype=np.float64) for x in (pos, vel, acc))
     out = orc.step(cfg.params, float(np.float32(cfg.dt)), p64, v64, a64, flags)
-    out.update(pos=p64, vel=v64, acc=a64)
+    out.update(pos=p64, vel=v64, acc=a64, colour=orc.colours(cfg.params, out["density"]))
     return out
 
 
@@ -90,16 +90,37 @@ def gpu_states(lf, cfg, checkpoints):
     return states
 
 
-def compare_substep(g, o, psize, label, tol=None):
-    """Assert bit-exact integer stages and TOL on every continuous field
-    (tol: per-field overrides, each justified where it is passed)."""
-    tol = tol or {}
+def derived_acc_floor(g, o, params):
+    """Acceleration error the measured force error allows, relative to the
+    largest acceleration.  Both sides integrate a' = clamp(m F + g) in FP64
+    (fluidintegrator.cu:41-50); the clamp is a projection (1-Lipschitz), so
+    |a'_gpu - a'_oracle| <= m |F_gpu - F_oracle| plus the FP32 rounding of
+    the stored acceleration and of the four stored force components.  Where
+    pair forces nearly cancel and the clamp at acc_limit caps the largest
+    accelerations, this exceeds TOL although the force meets it."""
+    m = float(np.float32(params.mass))
+    gF = g["f_pressure"] + g["f_viscosity"] + g["f_tension"] + g["f_adhesion"]
+    oF = o["f_pressure"] + o["f_viscosity"] + o["f_tension"] + o["f_adhesion"]
+    comp = sum(np.linalg.norm(np.asarray(g[k], np.float64), axis=1)
+               for k in ("f_pressure", "f_viscosity", "f_tension", "f_adhesion"))
+    allowed = m * (np.linalg.norm(gF - oF, axis=1) + 1e-6 * comp) + 1e-6 * np.linalg.norm(o["acc"], axis=1)
+    return float(allowed.max() / np.linalg.norm(o["acc"], axis=1).max()), allowed
+
+
+def compare_substep(g, o, psize, label, orc=None, cfg=None, state=None):
+    """Assert bit-exact integer stages and TOL normwise on every continuous
+    field.  Two derived fields may exceed TOL only by a bound computed here
+    from the oracle (orc, cfg and the FP32 input state let it recompute):
+    acc by the error the measured force error allows (derived_acc_floor,
+    checked particle by particle), vel by twice the FP32-storage floor of
+    XSPH (the oracle with its integrated positions rounded to FP32, as the
+    reference stores them, against the oracle)."""
     assert np.array_equal(g["keys"], o["keys"]), f"{label}: cell keys differ"
     assert np.array_equal(g["order"], o["order"]), f"{label}: sorted order differs"
     assert np.array_equal(g["start"], o["start"]), f"{label}: cell start table differs"
     errs = {}
     for k in ("density", "pressure", "sound_speed", "normal", "f_pressure", "f_viscosity", "f_tension",
-              "f_adhesion", "pos", "acc"):
+              "f_adhesion", "pos", "acc", "colour"):
         errs[k] = normwise(g[k], o[k])
     gF = g["f_pressure"] + g["f_viscosity"] + g["f_tension"] + g["f_adhesion"]
     oF = o["f_pressure"] + o["f_viscosity"] + o["f_tension"] + o["f_adhesion"]
@@ -108,17 +129,31 @@ def compare_substep(g, o, psize, label, tol=None):
     keep = ~band
     errs["vel_half"] = normwise(g["vel_half"][keep], o["vel_half"][keep])
     errs["vel"] = normwise(g["vel"][keep], o["vel"][keep])
-    # Relative L2 over all particles, held to TOL on every field (no overrides).
+    # Relative L2 over all particles, held to TOL on every field.
     l2 = {k: rel_l2(g[k], o[k]) for k in ("density", "pressure", "sound_speed", "normal", "f_pressure",
-                                           "f_viscosity", "f_tension", "f_adhesion", "pos", "acc")}
+                                           "f_viscosity", "f_tension", "f_adhesion", "pos", "acc", "colour")}
     l2["force"] = rel_l2(gF, oF)
     l2["vel_half"] = rel_l2(g["vel_half"][keep], o["vel_half"][keep])
     l2["vel"] = rel_l2(g["vel"][keep], o["vel"][keep])
+    bound = {k: TOL for k in errs}
+    if errs["acc"] > TOL:
+        assert cfg is not None, f"{label}: acc {errs['acc']:.3e} above TOL"
+        floor, allowed = derived_acc_floor(g, o, cfg.params)
+        over = np.linalg.norm(np.asarray(g["acc"], np.float64) - o["acc"], axis=1) > allowed
+        assert not over.any(), f"{label}: {int(over.sum())} accelerations differ by more than their force error allows"
+        bound["acc"] = max(TOL, floor)
+        print(f"[{label}] acc bound from the force error: {floor:.3e}")
+    if errs["vel"] > TOL:
+        assert orc is not None and state is not None, f"{label}: vel {errs['vel']:.3e} above TOL"
+        o32 = oracle_substep(orc, cfg, *state, flags=PIN_F32 | XSPH_F32_POS)
+        floor = normwise(o32["vel"][keep], o["vel"][keep])
+        bound["vel"] = max(TOL, 2.0 * floor)
+        print(f"[{label}] XSPH FP32-storage floor of vel: {floor:.3e}")
     print(f"\n[{label}] wall-threshold particles excluded from vel: {int(band.sum())}")
     for k, v in errs.items():
-        print(f"  {k:12s} normwise {v:.3e}  rel-L2 {l2[k]:.3e}")
-    bad = {k: v for k, v in errs.items() if not v <= tol.get(k, TOL)}
-    assert not bad, f"{label}: fields above {TOL} (overrides {tol}): {bad}"
+        print(f"  {k:12s} normwise {v:.3e} (bound {bound[k]:.1e})  rel-L2 {l2[k]:.3e}")
+    bad = {k: v for k, v in errs.items() if not v <= bound[k]}
+    assert not bad, f"{label}: fields above their bound: {bad}"
     bad = {k: v for k, v in l2.items() if not v <= TOL}
     assert not bad, f"{label}: relative L2 above {TOL}: {bad}"
 
@@ -498,21 +533,19 @@ def test_full_size_spray_c3(gpu, oracle_omp):
     print(f"\noccupied cells {occupied.size}, particles per occupied cell min {occupied.min()} max {occupied.max()}")
     o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
     # Density, forces and positions are held to TOL, and every field's
-    # relative L2.  Two derived fields get a wider max-norm bound here:
+    # relative L2.  Two derived fields get bounds computed from the oracle
+    # in compare_substep:
     # - vel: the post-XSPH velocity of an isolated spray particle with one or
     #   two partners moves by ~20% in XSPH, and that correction goes with
     #   (h^2 - r^2)^3 of the *integrated* positions, which are stored in FP32
-    #   (as in the reference) while the oracle keeps them in FP64: the ~6e-8
-    #   storage rounding gives ~3e-5 (the dense scenes stay below 1e-5);
+    #   (as in the reference) while the oracle keeps them in FP64 -- bounded
+    #   by twice the oracle's own change when it rounds them to FP32;
     # - acc: clamped at acc_limit (150), so its max-norm is relative to 150,
-    #   while the force's is relative to the spray's largest forces.  A spray
-    #   particle (density 0.03-0.3 rho0, three or four partners) whose pair
-    #   forces nearly cancel carries the FP32 rounding of its terms (h - r
-    #   near the kernel's edge: ~1e-6 relative, computed in FP32 as in the
-    #   reference) amplified by the cancellation: up to ~1.2e-5 of 150 on a
-    #   few particles, 3e-7 in relative L2.
-    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c3 replay from step 5",
-                    tol={"vel": 1e-4, "acc": 3e-5})
+    #   while the force's is relative to the spray's largest forces; a spray
+    #   particle whose pair forces nearly cancel carries its force error into
+    #   its acceleration -- bounded particle by particle by the force error.
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), "c3 replay from step 5", orc=oracle_omp,
+                    cfg=cfg, state=(pos, vel, acc))
 
 
 @pytest.mark.parametrize("case", ["c1", "double_dam", "dense", "packed", "cloud"])

@@ -130,6 +130,21 @@ def test_colour_is_continuous(oracle):
     assert list(oracle.colour(p, 0.0)) == [0.0, 0.0, 1.0]
 
 
+def test_colours_closed_form(oracle):
+    """DensityColourOperator (fluidutil.cu:67-80): r = rho/rho0; r < 1 ->
+    (0, r, 1-r); r < 2 -> (r-1, 2-r, 0); else (1, 0, 0); the array form
+    equals the scalar one."""
+    p = unit_params(16)
+    rho = np.array([0.0, 250.0, 999.0, 1000.0, 1500.0, 1999.0, 2000.0, 3500.0])
+    r = rho / 1000.0
+    want = np.where((r < 1)[:, None], np.stack([0 * r, r, 1 - r], 1),
+                    np.where((r < 2)[:, None], np.stack([r - 1, 2 - r, 0 * r], 1), np.array([1.0, 0.0, 0.0])))
+    got = oracle.colours(p, rho)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+    for k, d in enumerate(rho):
+        assert np.array_equal(oracle.colour(p, d), got[k])
+
+
 # ---- integer stages ----------------------------------------------------------------
 
 def test_hash_matches_numpy_fp32(oracle):
