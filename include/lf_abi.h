@@ -172,6 +172,27 @@ int lf_debug_flat_table(const uint64_t masks[9], const uint32_t first_slot[9], u
 int lf_debug_fail_alloc(int64_t nth);
 int lf_debug_live_allocs(int64_t *count);
 
+/* ---- stepping without allocating -------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: work recorded under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Runs `substeps` substeps of dt/substeps
+ * of one scene with parameters p on device arrays the caller owns: pos_xyz,
+ * vel_xyz and acc_xyz hold num_particles float3 (x,y,z interleaved) in
+ * particle-id order; they are read first and overwritten with the new state.
+ * The result is bitwise that of lf_create + lf_upload + lf_step(dt,
+ * substeps) + lf_download(LF_ORDER_ID) with the default launch shapes.  The
+ * call allocates nothing (no device, pinned or heap memory, no stream, event
+ * or graph): every working buffer lies in a fixed 32 MiB device array of the
+ * library, one per device, held from the first call on that device until the
+ * process ends.  That fits about 350 bytes per particle (the count rounded
+ * up to 512) plus 8 bytes per grid cell: e.g. 65,536 particles on up to
+ * 1,000,000 cells; larger scenes are rejected with LF_ERR_INVALID.  Work is
+ * only enqueued on `stream` (a hipStream_t; NULL: the default stream);
+ * completion is the stream's.  Calls share the device's array, so calls that
+ * overlap on one device (other streams or threads) race. */
+int lf_step_noalloc(const lf_params *p, uint32_t num_particles, float dt, uint32_t substeps, float *pos_xyz,
+                    float *vel_xyz, float *acc_xyz, void *stream);
+
 /* ---- timing ------------------------------------------------------------ */
 /* Kernel classes timed by lf_profile_*. */
 enum lf_kernel {

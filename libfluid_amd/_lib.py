@@ -195,6 +195,8 @@ def _load():
         "lf_debug_flat_table": (C.c_int, [P(C.c_uint64), u32p, u32p, u32p, u32p]),
         "lf_debug_fail_alloc": (C.c_int, [C.c_int64]),
         "lf_debug_live_allocs": (C.c_int, [P(C.c_int64)]),
+        "lf_step_noalloc": (C.c_int, [P(LfParams), C.c_uint32, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -221,7 +223,7 @@ EXPORTED = (
     "lf_checkpoint_save lf_checkpoint_load lf_setup_jittered_lattice lf_setup_fluid_block lf_export_geo_async "
     "lf_export_wait lf_batch_checkpoint_save lf_batch_checkpoint_load lf_batch_setup_jittered_lattice "
     "lf_batch_setup_fluid_block lf_batch_export_geo_async lf_batch_export_wait "
-    "lf_debug_flat_table lf_debug_fail_alloc lf_debug_live_allocs"
+    "lf_debug_flat_table lf_debug_fail_alloc lf_debug_live_allocs lf_step_noalloc"
 ).split()
 
 
