@@ -56,6 +56,9 @@ struct Consts {
     // density + EOS
     float mC6;  // m * 315/(64 pi h^9)
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
+    // gamma and gamma - 1 when they are integers in [1, 15], else 0
+    // (tait_pow then multiplies instead of calling powf)
+    int gamma_i, gamma_m1_i;
     // normals: lazy_normal = no force term reads this scene's normals (no
     // surface tension), so they are computed only when read
     float hmCg6;  // h * m * (-945/(32 pi h^9))
@@ -82,6 +85,25 @@ struct Consts {
 };
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+// ratio^e of the Tait EOS (fluidphysics.cu:78,82).  For an integer exponent
+// ei in [1, 15] by binary powering in FP64, rounded once to FP32 (branches on
+// ei are uniform: one scene per workgroup); otherwise powf.
+__device__ __forceinline__ float tait_pow(float ratio, float e, int ei) {
+    if (ei > 0) {
+        double x = (double)ratio;
+        double p = (ei & 1) ? x : 1.0;
+        x *= x;
+        if (ei & 2) p *= x;
+        if (ei >= 4) {
+            x *= x;
+            if (ei & 4) p *= x;
+            if (ei >= 8) p *= x * x;
+        }
+        return (float)p;
+    }
+    return powf(ratio, e);
+}
 
 // Squared distance with a fixed FMA order.  The cell-key and pair-predicate
 // decisions are made on this exact expression, which the oracle reproduces.
@@ -320,6 +342,18 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
         if (__ballot(v >= t)) mx = t;
     }
     return mx;
+}
+
+// Smallest v (v < 2^BITS) over the wave's active lanes, by BITS ballots.
+template <int BITS>
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+    uint32_t mn = 0u;
+#pragma unroll
+    for (int b = BITS - 1; b >= 0; --b) {
+        const uint32_t t = mn | (1u << b);
+        if (!__ballot(v < t)) mn = t;
+    }
+    return mn;
 }
 
 // The scene's quantised largest move of the substep (the XSPH skin): the

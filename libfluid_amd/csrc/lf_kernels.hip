@@ -622,6 +622,8 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     // n-1-k of the n slots scanned into it; one bit reversal per row moves it
     // to bit k.  Slots past 64 (rows without masks) are tested per lane.
     // Only the centre row (r = 4: dz = dy = 0) holds the particle itself.
+    // Slots below the wave's shortest row need no end-of-row test: there the
+    // term is max(h^2 - r^2, 0), the same value as the masked select.
     const __amdgpu_buffer_rsrc_t px = plane_buffer(soa, nslots), py = plane_buffer(soa + nslots, nslots),
                                  pz = plane_buffer(soa + 2 * (size_t)nslots, nslots);
 #pragma unroll
@@ -633,29 +635,41 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         // to a multiple of 4; slots past a lane's row add nothing.
         const uint32_t lw = (wave_max<7>(len < 64u ? len : 64u) + 3u) & ~3u;
         const uint32_t lw0 = lw < 32u ? lw : 32u;
+        // slots [0, lf) lie in every lane's row (lf <= lw)
+        const uint32_t lf = wave_min<7>(len < 64u ? len : 64u) & ~3u;
         uint32_t s0 = 0u, s1 = 0u;
-        auto test = [&](uint32_t k, float qx, float qy, float qz, uint32_t &sm) {
+        auto test = [&]<bool FULL>(uint32_t k, float qx, float qy, float qz, uint32_t &sm) {
             const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
-            uint64_t live = lanes(k < len);
-            if (r == 4) live &= ~lanes(k == kself);
-            const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
-            const float t = keep(c.h2 - rr, in);
-            S += t * t * t;
-            sm = shift_in(sm, ins);
+            if (FULL && r != 4) {
+                const float t = fmaxf(c.h2 - rr, 0.0f);
+                S += t * t * t;
+                sm = shift_in(sm, lanes(rr <= h2s));
+            } else {
+                uint64_t live = lanes(k < len);
+                if (r == 4) live &= ~lanes(k == kself);
+                const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
+                const float t = keep(c.h2 - rr, in);
+                S += t * t * t;
+                sm = shift_in(sm, ins);
+            }
         };
         // Slots k0 .. k0 + 3 (every lane past its row's end for all four
         // when k0 >= its length).
-        auto four = [&](uint32_t k0, uint32_t &sm) {
+        auto four = [&]<bool FULL>(uint32_t k0, uint32_t &sm) {
             const uint32_t so = k0 * (uint32_t)sizeof(float);
             const float4 X = load4(px, off, so), Y = load4(py, off, so), Z = load4(pz, off, so);
-            test(k0, X.x, Y.x, Z.x, sm);
-            test(k0 + 1u, X.y, Y.y, Z.y, sm);
-            test(k0 + 2u, X.z, Y.z, Z.z, sm);
-            test(k0 + 3u, X.w, Y.w, Z.w, sm);
+            test.template operator()<FULL>(k0, X.x, Y.x, Z.x, sm);
+            test.template operator()<FULL>(k0 + 1u, X.y, Y.y, Z.y, sm);
+            test.template operator()<FULL>(k0 + 2u, X.z, Y.z, Z.z, sm);
+            test.template operator()<FULL>(k0 + 3u, X.w, Y.w, Z.w, sm);
         };
-        for (uint32_t k = 0; k < lw0; k += 4u) four(k, s0);
-        for (uint32_t k = 32u; k < lw; k += 4u) four(k, s1);
-        for (uint32_t k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
+        const uint32_t f0 = lf < lw0 ? lf : lw0, f1 = lf > 32u ? lf : 32u;
+        uint32_t k = 0u;
+        for (; k < f0; k += 4u) four.template operator()<true>(k, s0);
+        for (; k < lw0; k += 4u) four.template operator()<false>(k, s0);
+        for (k = 32u; k < f1; k += 4u) four.template operator()<true>(k, s1);
+        for (; k < lw; k += 4u) four.template operator()<false>(k, s1);
+        for (k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
             const float rr = dist2(pi, at32(Ps, bb + k));
             const bool in = rr <= c.h2 && (r != 4 || k != kself);
             const float t = in ? c.h2 - rr : 0.0f;
@@ -714,8 +728,8 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     }
     const float rho = (S == 0.0f) ? c.rho0 : corrected;
     const float ratio = corrected * c.inv_rho0;
-    const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (powf(ratio, c.gamma) - 1.0f);
-    const float cs_i = c.K * powf(ratio, c.gamma_m1);
+    const float press = (corrected == 0.0f) ? 0.0f : c.eos_scale * (tait_pow(ratio, c.gamma, c.gamma_i) - 1.0f);
+    const float cs_i = c.K * tait_pow(ratio, c.gamma_m1, c.gamma_m1_i);
     const float ir = 1.0f / rho;
     // Every pair the later passes treat as near-isolated (1/rho_j or 1/rho_i
     // above fragile_ir; rho_i + rho_j below fragile_sr, which needs one
@@ -842,6 +856,8 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
         __shared__ uint2 tab[kFlatEntries * BS];
         const bool fragile_i = iri > c.fragile_ir;
         auto walk = [&]<bool V, bool T, bool F>() {
+            // (with viscosity the neighbour's pressure comes with its
+            // density record, D.y, instead of a load of its own)
             struct Cand {
                 float4 q;
                 float p;
@@ -854,10 +870,12 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                 [&](uint32_t j) {
                     Cand cd;
                     cd.q = at32(Q, j);
-                    cd.p = at32(Pr, j);
                     if constexpr (V) {
                         cd.v = at32(Vs, j);
                         cd.d = at32(D, j);
+                        cd.p = cd.d.y;
+                    } else {
+                        cd.p = at32(Pr, j);
                     }
                     if constexpr (T) cd.n = at32(N, j);
                     return cd;

@@ -100,6 +100,9 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.eos_scale = (float)((double)p.gas_constant * (double)p.rest_density * inv_gamma);
     c.gamma = p.gamma;
     c.gamma_m1 = p.gamma - 1.0f;
+    const auto small_int = [](float e) { return (e >= 1.0f && e <= 15.0f && e == std::floor(e)) ? (int)e : 0; };
+    c.gamma_i = small_int(c.gamma);
+    c.gamma_m1_i = small_int(c.gamma_m1);
     c.K = p.gas_constant;
     c.bd_rho0_d = (double)p.rest_density * (double)0.0039062f;  // fluidkernel.cu:65
     c.h_d = h;

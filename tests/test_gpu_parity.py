@@ -172,6 +172,23 @@ def test_replay_plain_c1(gpu, oracle_omp, c1, step):
     compare_substep(g, o, float(np.float32(c1.params.particle_size)), f"c1 replay from step {step}")
 
 
+@pytest.mark.parametrize("gamma", [1.0, 2.0, 7.5, 15.0])
+def test_replay_eos_exponents(gpu, oracle_omp, gamma):
+    """Tait exponents other than 7 (fluidphysics.cu:78,82): integer ones
+    (1, 2, 15; gamma - 1 = 0 leaves the sound speed at K) take the kernel's
+    exact powering, 7.5 takes powf.  With viscosity on in the last case, so
+    that the force walk takes the neighbour pressure from its density record."""
+    cfg = configs.dambreak_c1()
+    cfg.params.gamma = gamma
+    if gamma == 15.0:
+        cfg.params.viscosity = 0.2
+    pos, vel, acc = gpu_states(gpu, cfg, [10])[10]
+    g = gpu_substep(gpu, cfg, pos, vel, acc)
+    o = oracle_substep(oracle_omp, cfg, pos, vel, acc)
+    compare_substep(g, o, float(np.float32(cfg.params.particle_size)), f"c1 gamma {gamma} replay from step 10",
+                    orc=oracle_omp, cfg=cfg, state=(pos, vel, acc))
+
+
 def test_replay_double_dam_preset(gpu, oracle_omp):
     """Reference double dam break with a=0.1, v=0.2, s=1, d=0.5 (every force term on)."""
     cfg = configs.double_dam(n=4096, res=16)
