@@ -70,8 +70,8 @@ void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool 
 void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
                    const float *Pr, const float4 *Vs, const float4 *A, const float4 *Lw, const uint32_t *order,
                    const float4 *D, const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout,
-                   float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
-void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *X,
+                   float4 *Aout, float4 *Lout, float4 *Fdbg, uint32_t *flags, const uint32_t *moves);
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Pn,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout);
 

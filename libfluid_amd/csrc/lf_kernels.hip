@@ -806,9 +806,9 @@ __global__ __launch_bounds__(BS) void k_normals(const Consts *__restrict__ cs, c
 // leapfrog integrator and boundary clamp (LeapfrogIntegratorOperator,
 // fluidintegrator.cu:11-70).  Out-of-grid particles get zero forces and are
 // still integrated.  Fdbg (may be null) receives the four force components.
-// Writes X = (integrated position, density) for the XSPH pass and sets the
-// scene's far-move flag when a particle moved farther than the XSPH skin
-// masks allow.
+// The pre-XSPH velocity carries the particle's density in w for the XSPH
+// pass, which reads the integrated positions from Pout.  Sets the scene's
+// far-move flag when a particle moved farther than the XSPH skin masks allow.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                uint32_t nslots, const float4 *__restrict__ Ps,
@@ -819,7 +819,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                                                const float4 *__restrict__ N, const uint32_t *__restrict__ start,
                                                RowMasks M, int flat, float4 *__restrict__ Pout,
                                                float4 *__restrict__ Vout, float4 *__restrict__ Aout,
-                                               float4 *__restrict__ Lout, float4 *__restrict__ X,
+                                               float4 *__restrict__ Lout,
                                                float4 *__restrict__ Fdbg, uint32_t *__restrict__ flags,
                                                const uint32_t *__restrict__ moves, const uint8_t *__restrict__ near) {
     uint32_t slot0;
@@ -1039,9 +1039,8 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
     const float hx = (float)x, hy = (float)y, hz = (float)z;
     Pout[i] = make_float4(hx, hy, hz, pi.w);
     Lout[i] = make_float4((float)(x - (double)hx), (float)(y - (double)hy), (float)(z - (double)hz), 0.0f);
-    Vout[i] = make_float4((float)vx, (float)vy, (float)vz, 0.0f);
+    Vout[i] = make_float4((float)vx, (float)vy, (float)vz, di.x);
     Aout[i] = make_float4((float)ax, (float)ay, (float)az, 0.0f);
-    X[i] = make_float4(hx, hy, hz, di.x);
     // Move of the FP32 positions the pair predicates see (the differences
     // are exact in FP64).  Every writer stores the same value; NaN counts
     // as too far.
@@ -1051,8 +1050,9 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
 }
 
 // 8. XSPH velocity smoothing (correctVelocityXSPH, fluidphysics.cu:355-409):
-// the stencil of the pre-integration cell, integrated positions, and the
-// pre-XSPH velocity of every neighbour (Jacobi update into Vout).  The skin
+// the stencil of the pre-integration cell (Ps), integrated positions (Pn),
+// and the pre-XSPH velocity of every neighbour, with its density in w (Vh;
+// Jacobi update into Vout).  The skin
 // masks hold every stencil slot within the skin radius h_s before the move;
 // a pair within h after the move was within h + 2 max|dx| before it, so the
 // masks are complete while no particle moved farther than move_limit
@@ -1061,7 +1061,7 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
 template <int BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                              uint32_t nslots, const float4 *__restrict__ Ps,
-                                             const float4 *__restrict__ X, const float4 *__restrict__ Vh,
+                                             const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
                                              const float4 *__restrict__ Lw, const uint32_t *__restrict__ start,
                                              RowMasks MS, int flat, const uint32_t *__restrict__ flags,
                                              const uint8_t *__restrict__ near,
@@ -1076,13 +1076,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t shift = c.first - st[0];
     const float4 vi = Vh[i];
     if (i >= st[c.ncells] + shift) {
-        Vout[i] = vi;
+        Vout[i] = make_float4(vi.x, vi.y, vi.z, 0.0f);
         return;
     }
     const uint32_t fl = flag_word(flags, c.scene);
     const bool masks_ok = c.use_masks && !flag_set(fl, kFarMoveByte);
-    const float4 xi = X[i];
-    const float rho_i = xi.w;
+    const float4 xi = Pn[i];
+    const float rho_i = vi.w;
     uint32_t b[9], e[9];
     rows9(c, st, shift, Ps[i], b, e);
     float sx = 0.f, sy = 0.f, sz = 0.f;
@@ -1098,12 +1098,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     auto walk = [&]<bool F>() {
         walk_rows<BS, F ? 2 : kXsphWalkWidth, F ? 2 : kXsphRowWidth>(
             tab, b, e, MS, masks_ok, flat, nslots, i,
-            [&](uint32_t j) { return Cand{at32(X, j), at32(Vh, j)}; },
+            [&](uint32_t j) { return Cand{at32(Pn, j), at32(Vh, j)}; },
             [&](const Cand &cj, uint32_t j, bool ok) {
                 const float4 q = cj.x;
                 const float4 vj = cj.v;
                 const float rr = dist2(xi, q);
-                const float sr = rho_i + q.w;
+                const float sr = rho_i + vj.w;
                 const uint64_t in = lanes(rr <= h2) & lanes(sr > eps) & lanes(ok);
                 float t = keep(h2 - rr, in);
                 if constexpr (F) {
@@ -1435,18 +1435,18 @@ void launch_normals(const Launch &L, const PoolView &pv, int bs, int flat, bool 
 void launch_forces(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Q,
                    const float *Pr, const float4 *Vs, const float4 *A, const float4 *Lw, const uint32_t *order,
                    const float4 *D, const float4 *N, const uint32_t *start, RowMasks M, float4 *Pout, float4 *Vout,
-                   float4 *Aout, float4 *Lout, float4 *X, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
+                   float4 *Aout, float4 *Lout, float4 *Fdbg, uint32_t *flags, const uint32_t *moves) {
     with_bs(bs, [&]<int BS>() {
         LF_GO(L, k_forces<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Q, Pr, Vs, A, Lw, order,
-              D, N, start, M, flat, Pout, Vout, Aout, Lout, X, Fdbg, flags, moves, pv.near);
+              D, N, start, M, flat, Pout, Vout, Aout, Lout, Fdbg, flags, moves, pv.near);
     });
 }
 
-void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *X,
+void launch_xsph(const Launch &L, const PoolView &pv, int bs, int flat, const float4 *Ps, const float4 *Pn,
                  const float4 *Vh, const float4 *Lw, const uint32_t *start, RowMasks MS, const uint32_t *flags,
                  float4 *Vout) {
     with_bs(bs, [&]<int BS>() {
-        LF_GO(L, k_xsph<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, X, Vh, Lw, start, MS,
+        LF_GO(L, k_xsph<BS>, pv.nlaunch / BS, BS, L.a, L.b, pv.cs, pv.chunk, pv.nslots, Ps, Pn, Vh, Lw, start, MS,
               flat, flags, pv.near, Vout);
     });
 }

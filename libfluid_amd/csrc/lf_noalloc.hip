@@ -58,7 +58,8 @@ int scratch_base(unsigned char **out) {
 // buckets (its cells plus the out-of-grid bucket), as the pool lays them
 // out, carved in order from a base address with 256-byte alignment.
 struct Buffers {
-    float4 *P0, *P1, *V0, *V1, *V2, *A0, *A1, *L0, *L1, *D, *N, *Q, *X;
+    float4 *P0, *P1, *V0, *V1, *V2, *A0, *A1, *L0, *L1, *D, *N, *Q;
+    uint2 *segtmp;
     float *Pr, *soa;
     RowMasks MS;
     uint32_t *keys, *ranks, *slot_src, *slot_id, *start, *bsum, *chunk;
@@ -96,8 +97,9 @@ size_t carve(unsigned char *base, uint32_t slots, uint32_t buckets, Buffers &b) 
     b.near = c.take<uint8_t>(n / kNearBlock);
     b.zero_begin = b.counts;
     b.zero_words = (c.off - (size_t)((unsigned char *)b.counts - base) + 3) / 4;
-    float4 **f4[] = {&b.P0, &b.P1, &b.V0, &b.V1, &b.V2, &b.A0, &b.A1, &b.L0, &b.L1, &b.D, &b.N, &b.Q, &b.X};
+    float4 **f4[] = {&b.P0, &b.P1, &b.V0, &b.V1, &b.V2, &b.A0, &b.A1, &b.L0, &b.L1, &b.D, &b.N, &b.Q};
     for (float4 **p : f4) *p = c.take<float4>(n);
+    b.segtmp = c.take<uint2>(n);
     b.Pr = c.take<float>(n);
     b.soa = c.take<float>(3 * n);
     b.MS.w32 = c.take<uint32_t>(9 * n + 2);  // +2: an 8-byte read at the last entry stays inside
@@ -204,15 +206,14 @@ extern "C" int lf_step_noalloc(const lf_params *p, uint32_t num_particles, float
         // One substep, the pool's sequence (Pool::enqueue_sort, enqueue_substep).
         launch_hash(L, pv, b.P0, b.keys, b.ranks, b.counts, b.flags, b.moves, b.nsegs);
         launch_scan(L, b.counts, buckets, b.bsum, b.start);
-        launch_scatter(L, pv, b.P0, b.keys, b.ranks, b.start, b.slot_src, b.slot_id, b.segs, b.nsegs,
-                       reinterpret_cast<uint2 *>(b.X));
+        launch_scatter(L, pv, b.P0, b.keys, b.ranks, b.start, b.slot_src, b.slot_id, b.segs, b.nsegs, b.segtmp);
         launch_gather(L, pv, b.slot_src, b.slot_id, b.keys, b.start, b.P0, b.V0, b.P1, b.V1, b.ranks, b.soa, b.moves);
         launch_density(L, pv, t.density, b.P1, b.soa, b.start, b.D, b.Q, b.Pr, b.MS, b.flags, b.moves);
         if (!c.lazy_normal)
             launch_normals(L, pv, t.normals, t.flat_normals, false, b.Q, b.start, b.MS, b.flags, b.N);
         launch_forces(L, pv, t.forces, t.flat_forces, b.P1, b.Q, b.Pr, b.V1, b.A0, b.L0, b.ranks, b.D, b.N, b.start,
-                      b.MS, b.P0, b.V2, b.A1, b.L1, b.X, nullptr, b.flags, b.moves);
-        launch_xsph(L, pv, t.xsph, t.flat_xsph, b.P1, b.X, b.V2, b.L1, b.start, b.MS, b.flags, b.V0);
+                      b.MS, b.P0, b.V2, b.A1, b.L1, nullptr, b.flags, b.moves);
+        launch_xsph(L, pv, t.xsph, t.flat_xsph, b.P1, b.P0, b.V2, b.L1, b.start, b.MS, b.flags, b.V0);
         std::swap(b.A0, b.A1);
         std::swap(b.L0, b.L1);
     }

@@ -162,7 +162,8 @@ class Pool {
     // device arrays: state (P0,V0,A0,L0) and the substep's working arrays
     float4 *P0_ = nullptr, *P1_ = nullptr, *V0_ = nullptr, *V1_ = nullptr, *V2_ = nullptr;
     float4 *A0_ = nullptr, *A1_ = nullptr, *L0_ = nullptr, *L1_ = nullptr;
-    float4 *D_ = nullptr, *N_ = nullptr, *F_ = nullptr, *Q_ = nullptr, *X_ = nullptr;
+    float4 *D_ = nullptr, *N_ = nullptr, *F_ = nullptr, *Q_ = nullptr;
+    uint2 *segtmp_ = nullptr;  // the segment sort's scratch, one entry per slot
     float *Pr_ = nullptr;
     float *soa_ = nullptr;  // sorted positions as three planes of nslots floats (x, y, z)
     RowMasks MS_{nullptr, nullptr};  // neighbour masks at the skin radius (density pass)

@@ -154,8 +154,9 @@ void Pool::release() {
     drop_graph();
     dev_free(MS_.w32);
     dev_free(MS_.w64);
-    float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &F_, &Q_, &X_};
+    float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &F_, &Q_};
     for (float4 **b : f4) dev_free(*b);
+    dev_free(segtmp_);
     dev_free(Pr_);
     dev_free(soa_);
     uint32_t **u32[] = {&keys_, &ranks_, &slot_src_, &slot_id_, &counts_, &start_, &bsum_, &nsegs_, &flags_,
@@ -206,9 +207,10 @@ int Pool::init(uint32_t slot_cap, uint32_t bucket_cap, uint32_t scene_cap) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(LF_ERR_HIP, "pool: %s", hipGetErrorString(e));
     const size_t n = nslots_;
-    float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &Q_, &X_};
+    float4 **f4[] = {&P0_, &P1_, &V0_, &V1_, &V2_, &A0_, &A1_, &L0_, &L1_, &D_, &N_, &Q_};
     for (float4 **b : f4)
         if (!rc) rc = dalloc(b, n);
+    if (!rc) rc = dalloc(&segtmp_, n);
     if (!rc) rc = dalloc(&Pr_, n);
     if (!rc) rc = dalloc(&soa_, 3 * n);
     if (!rc) rc = dalloc(&MS_.w32, 9 * n + 2);  // +2: an 8-byte read at the last entry stays inside
@@ -538,10 +540,8 @@ void Pool::enqueue_sort() {
         launch_hash(L, pv, P0_, keys_, ranks_, counts_, flags_, moves_, nsegs_);
     });
     timed(LF_K_SCAN, [&](const Launch &L) { launch_scan(L, counts_, span_buckets_, bsum_, start_); });
-    // X_ (written by the force pass) is the segment sort's scratch.
     timed(LF_K_SCATTER, [&](const Launch &L) {
-        launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_, segs_, nsegs_,
-                       reinterpret_cast<uint2 *>(X_));
+        launch_scatter(L, pv, P0_, keys_, ranks_, start_, slot_src_, slot_id_, segs_, nsegs_, segtmp_);
     });
     timed(LF_K_GATHER, [&](const Launch &L) {
         launch_gather(L, pv, slot_src_, slot_id_, keys_, start_, P0_, V0_, P1_, V1_, ranks_, soa_, moves_);
@@ -550,7 +550,7 @@ void Pool::enqueue_sort() {
 
 // One substep of every active scene.  Buffer roles: state (P0,V0,A0,L0) ->
 // sorted (P1,V1) -> integrated P0, A1, L1 (cell order; A0 and L0 read
-// through ranks_) and pre-XSPH velocity V2 -> XSPH velocity V0.  The new
+// through ranks_) and pre-XSPH velocity V2 (density in w) -> XSPH velocity V0.  The new
 // acceleration and low position words are in A1 and L1: advance_state()
 // makes them the state.
 void Pool::enqueue_substep() {
@@ -565,10 +565,10 @@ void Pool::enqueue_substep() {
         });
     timed(LF_K_FORCES, [&](const Launch &L) {
         launch_forces(L, pv, tune_.forces, tune_.flat_forces, P1_, Q_, Pr_, V1_, A0_, L0_, ranks_, D_, N_, start_,
-                      MS_, P0_, V2_, A1_, L1_, X_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
+                      MS_, P0_, V2_, A1_, L1_, (debug_ & LF_DEBUG_FORCES) ? F_ : nullptr, flags_, moves_);
     });
     timed(LF_K_XSPH, [&](const Launch &L) {
-        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, X_, V2_, L1_, start_, MS_, flags_, V0_);
+        launch_xsph(L, pv, tune_.xsph, tune_.flat_xsph, P1_, P0_, V2_, L1_, start_, MS_, flags_, V0_);
     });
     for (const auto &r : paused_ranges()) launch_carry(stream_, r.first, r.second, A0_, L0_, A1_, L1_);
 }

@@ -300,11 +300,12 @@ int lf_export_wait(lf_scene *s) {
 int lf_batch_footprint(uint32_t max_particles, uint32_t max_cells, uint32_t max_scenes, uint64_t *bytes) {
     if (!bytes) return fail(LF_ERR_INVALID, "null argument");
     const uint64_t slots = ((uint64_t)max_particles + lf::kChunk - 1) / lf::kChunk * lf::kChunk;
-    // 13 float4 arrays, pressure, sorted positions as three float planes,
-    // 9 x (32 + 64)-bit masks, four uint32 sort arrays per slot;
+    // 12 float4 arrays, the segment sort's 8-byte scratch, pressure, sorted
+    // positions as three float planes, 9 x (32 + 64)-bit masks, four uint32
+    // sort arrays per slot;
     // counts and starts per bucket; per scene its constants, flag word and
     // move table (kMoveSlots words one cache line apart).
-    const uint64_t per_slot = 13 * 16 + 4 + 3 * 4 + 9 * (4 + 8) + 4 * 4;
+    const uint64_t per_slot = 12 * 16 + 8 + 4 + 3 * 4 + 9 * (4 + 8) + 4 * 4;
     const uint64_t per_scene = sizeof(lf::Consts) + 4 * lf::kSceneWords + 4 * lf::kMoveSlots * lf::kMoveStride;
     // plus the chunk map, near-isolated marks, and the sort's segment list
     // (one 16-byte entry per kRankDirectMax + 1 slots) and its count

@@ -155,7 +155,7 @@ def test_checkpoint_valid_file_reaches_scene_creation(tmp_path):
 
 
 def test_batch_footprint():
-    """Bytes a batch allocates: 348 per slot (slots rounded up to 512), 8 per
+    """Bytes a batch allocates: 340 per slot (slots rounded up to 512), 8 per
     bucket, per scene its table entry, flag word and 2 KB move table (16
     slots a cache line apart), per 512-slot chunk 4 for its map entry and 8
     near-isolated mark bytes (one per 64 slots), and 16 per 513 slots for
@@ -164,7 +164,7 @@ def test_batch_footprint():
     b2 = lf.Batch.footprint(512, 10, 1)
     assert b1 == b2  # one chunk either way
     b3 = lf.Batch.footprint(1024, 10, 1)
-    assert b3 - b2 == 512 * 348 + 4 + 8 + 16
+    assert b3 - b2 == 512 * 340 + 4 + 8 + 16
     assert lf.Batch.footprint(512, 20, 1) - b2 == 80
     per_scene = lf.Batch.footprint(512, 10, 2) - b2
     assert 2048 + 200 < per_scene < 2048 + 1024
