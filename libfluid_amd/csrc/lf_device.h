@@ -344,18 +344,6 @@ __device__ __forceinline__ uint32_t wave_max(uint32_t v) {
     return mx;
 }
 
-// Smallest v (v < 2^BITS) over the wave's active lanes, by BITS ballots.
-template <int BITS>
-__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
-    uint32_t mn = 0u;
-#pragma unroll
-    for (int b = BITS - 1; b >= 0; --b) {
-        const uint32_t t = mn | (1u << b);
-        if (!__ballot(v < t)) mn = t;
-    }
-    return mn;
-}
-
 // The scene's quantised largest move of the substep (the XSPH skin): the
 // largest of its kMoveSlots slots (uniform addresses: scalar loads).
 __device__ __forceinline__ uint32_t scene_move_q(const uint32_t *__restrict__ moves, uint32_t scene) {

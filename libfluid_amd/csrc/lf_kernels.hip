@@ -611,19 +611,18 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     rows9(c, st, shift, pi, b, e);
     float S = 0.0f;
     // Branch-free candidate tests: a candidate outside h (or the particle
-    // itself, or past the end of the lane's row) adds an exact zero to S.
-    // The first 64 slots of every lane's row are scanned in wave-uniform
-    // loops up to the wave's longest row (slots 0..31, then 32..63), four
-    // slots per step: the sorted positions are also kept as x, y and z
-    // planes, so one 16-byte load per plane brings four candidates.  The
-    // slot offset k is a scalar, so a load's address is the lane's row offset
-    // plus a scalar (buffer soffset).  Each mask word is built as m = 2m +
-    // bit, one add-with-carry per candidate, which leaves slot k at bit
-    // n-1-k of the n slots scanned into it; one bit reversal per row moves it
-    // to bit k.  Slots past 64 (rows without masks) are tested per lane.
-    // Only the centre row (r = 4: dz = dy = 0) holds the particle itself.
-    // Slots below the wave's shortest row need no end-of-row test: there the
-    // term is max(h^2 - r^2, 0), the same value as the masked select.
+    // itself, or past the end of the lane's row) adds an exact zero to S;
+    // in a whole group of four the term is max(h^2 - r^2, 0), the same value.
+    // The first 64 slots of every lane's row are scanned four slots per step
+    // (slots 0..31 into one mask word, 32..63 into another): the sorted
+    // positions are also kept as x, y and z planes, so one 16-byte load per
+    // plane brings four candidates.  The slot offset k is a scalar, so a
+    // load's address is the lane's row offset plus a scalar (buffer
+    // soffset).  Each mask word is built as m = 2m + bit, one add-with-carry
+    // per candidate, which leaves slot k at bit n-1-k of the n slots scanned
+    // into it; one bit reversal per row moves it to bit k.  Slots past 64
+    // (rows without masks) are tested per lane.  Only the centre row (r = 4:
+    // dz = dy = 0) holds the particle itself.
     const __amdgpu_buffer_rsrc_t px = plane_buffer(soa, nslots), py = plane_buffer(soa + nslots, nslots),
                                  pz = plane_buffer(soa + 2 * (size_t)nslots, nslots);
 #pragma unroll
@@ -633,42 +632,63 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         const uint32_t kself = i - bb;  // < len only in the centre row
         // Slots scanned: up to the wave's longest row (at most 64), rounded up
         // to a multiple of 4; slots past a lane's row add nothing.
-        const uint32_t lw = (wave_max<7>(len < 64u ? len : 64u) + 3u) & ~3u;
-        const uint32_t lw0 = lw < 32u ? lw : 32u;
-        // slots [0, lf) lie in every lane's row (lf <= lw)
-        const uint32_t lf = wave_min<7>(len < 64u ? len : 64u) & ~3u;
-        uint32_t s0 = 0u, s1 = 0u;
-        auto test = [&]<bool FULL>(uint32_t k, float qx, float qy, float qz, uint32_t &sm) {
+        // Slots with mask bits: [0, lm); whole groups of four: [0, nf).  The
+        // group loop's bound is per lane (a lane leaves it when its row's
+        // whole groups are done), so a group's four tests need no end-of-row
+        // test; the last, partial group tests each slot against the row's
+        // end.
+        const uint32_t lm = len < 64u ? len : 64u, nf = lm & ~3u;
+        uint32_t cur = 0u, s0 = 0u;  // the mask word being built; slots 0..31 once past them
+        auto test = [&]<bool FULL>(uint32_t k, float qx, float qy, float qz) {
             const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
             if (FULL && r != 4) {
                 const float t = fmaxf(c.h2 - rr, 0.0f);
                 S += t * t * t;
-                sm = shift_in(sm, lanes(rr <= h2s));
+                cur = shift_in(cur, lanes(rr <= h2s));
             } else {
                 uint64_t live = lanes(k < len);
                 if (r == 4) live &= ~lanes(k == kself);
                 const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
                 const float t = keep(c.h2 - rr, in);
                 S += t * t * t;
-                sm = shift_in(sm, ins);
+                cur = shift_in(cur, ins);
             }
         };
-        // Slots k0 .. k0 + 3 (every lane past its row's end for all four
-        // when k0 >= its length).
-        auto four = [&]<bool FULL>(uint32_t k0, uint32_t &sm) {
-            const uint32_t so = k0 * (uint32_t)sizeof(float);
-            const float4 X = load4(px, off, so), Y = load4(py, off, so), Z = load4(pz, off, so);
-            test.template operator()<FULL>(k0, X.x, Y.x, Z.x, sm);
-            test.template operator()<FULL>(k0 + 1u, X.y, Y.y, Z.y, sm);
-            test.template operator()<FULL>(k0 + 2u, X.z, Y.z, Z.z, sm);
-            test.template operator()<FULL>(k0 + 3u, X.w, Y.w, Z.w, sm);
+        auto group = [&]<bool FULL>(float4 X, float4 Y, float4 Z, uint32_t k0) {
+            if (k0 == 32u) {  // the low word is complete
+                s0 = cur;
+                cur = 0u;
+            }
+            test.template operator()<FULL>(k0, X.x, Y.x, Z.x);
+            test.template operator()<FULL>(k0 + 1u, X.y, Y.y, Z.y);
+            test.template operator()<FULL>(k0 + 2u, X.z, Y.z, Z.z);
+            test.template operator()<FULL>(k0 + 3u, X.w, Y.w, Z.w);
         };
-        const uint32_t f0 = lf < lw0 ? lf : lw0, f1 = lf > 32u ? lf : 32u;
+        // Two whole groups per step, all six loads issued before the tests.
         uint32_t k = 0u;
-        for (; k < f0; k += 4u) four.template operator()<true>(k, s0);
-        for (; k < lw0; k += 4u) four.template operator()<false>(k, s0);
-        for (k = 32u; k < f1; k += 4u) four.template operator()<true>(k, s1);
-        for (; k < lw; k += 4u) four.template operator()<false>(k, s1);
+        for (; k + 4u < nf; k += 8u) {
+            const uint32_t so = k * (uint32_t)sizeof(float), so4 = so + 4u * (uint32_t)sizeof(float);
+            const float4 AX = load4(px, off, so), AY = load4(py, off, so), AZ = load4(pz, off, so);
+            const float4 BX = load4(px, off, so4), BY = load4(py, off, so4), BZ = load4(pz, off, so4);
+            group.template operator()<true>(AX, AY, AZ, k);
+            group.template operator()<true>(BX, BY, BZ, k + 4u);
+        }
+        if (k < nf) {
+            const uint32_t so = k * (uint32_t)sizeof(float);
+            group.template operator()<true>(load4(px, off, so), load4(py, off, so), load4(pz, off, so), k);
+        }
+        if (nf < lm) {  // the partial group
+            const uint32_t so = nf * (uint32_t)sizeof(float);
+            group.template operator()<false>(load4(px, off, so), load4(py, off, so), load4(pz, off, so), nf);
+        }
+        // The lane scanned lw = lm rounded up to 4 slots: n0 = min(lw, 32)
+        // of them into the low word, lw - 32 into the high word.
+        const uint32_t lw = (lm + 3u) & ~3u, n0 = lw < 32u ? lw : 32u;
+        uint32_t s1 = 0u;
+        if (lw > 32u)
+            s1 = cur;
+        else
+            s0 = cur;
         for (k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
             const float rr = dist2(pi, at32(Ps, bb + k));
             const bool in = rr <= c.h2 && (r != 4 || k != kself);
@@ -676,13 +696,11 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             S += t * t * t;
         }
         const size_t at = (size_t)r * nslots + i;
-        const uint32_t sh0 = 32u - lw0;  // lw0 == 0 leaves s0 = 0
-        s0 = lw0 ? __brev(s0) >> sh0 : 0u;
+        s0 = n0 ? __brev(s0) >> (32u - n0) : 0u;  // n0 == 0: nothing scanned
         if (len <= 32u) {
             MS.w32[at] = s0;
         } else if (len <= 64u) {
-            const uint32_t sh1 = 64u - lw;  // lw - 32 slots scanned into the high word (lw >= len > 32)
-            MS.w64[at] = (uint64_t)(__brev(s1) >> sh1) << 32 | s0;
+            MS.w64[at] = (uint64_t)(__brev(s1) >> (64u - lw)) << 32 | s0;  // lw >= len > 32
         }
     }
     if (S < c.fragile_S) {
