@@ -229,7 +229,7 @@ __device__ __forceinline__ double skin_move_limit(const Consts &c, float s) {
 // exact, so the result is accurate to FP64 rounding.
 __device__ __forceinline__ double dist2_d(float4 a, float4 b) {
     const double dx = (double)b.x - (double)a.x, dy = (double)b.y - (double)a.y, dz = (double)b.z - (double)a.z;
-    return dx * dx + dy * dy + dz * dz;
+    return __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
 }
 
 // Cell coordinate along one axis: floor(x * res) (fluidutil.cu:15-18).

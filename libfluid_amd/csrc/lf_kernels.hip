@@ -434,13 +434,17 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // up to W set bits of the row and issues all their candidates' loads (ld)
 // before using them (fn, in slot order), so W gathers per lane are in flight
 // at a time.  fn(cand, slot, ok) adds nothing when ok is false (a step with
-// fewer set bits left passes its first candidate again with ok = false);
+// fewer set bits left passes its first slot again with ok = false and, as
+// that candidate, whatever finite values the previous step loaded there);
 // calling it unconditionally keeps the later candidates' loads ahead of the
 // first candidate's arithmetic.
 template <int W, class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
                                            uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
     uint64_t mk[9];
+    // A step's candidates past the lane's last set bit (ok = false) load
+    // nothing and keep the previous step's values (finite: zero at first).
+    decltype(ld(0u)) a[W] = {};
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         const uint32_t len = e[r] - b[r];
@@ -465,9 +469,10 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
                     j[k] = ok[k] ? b[r] + (uint32_t)__ffsll((unsigned long long)m) - 1u : j[0];
                     m &= m - 1ull;  // stays 0 when it was 0
                 }
-                decltype(ld(0u)) a[W];
+                a[0] = ld(j[0]);
 #pragma unroll
-                for (int k = 0; k < W; ++k) a[k] = ld(j[k]);
+                for (int k = 1; k < W; ++k)
+                    if (ok[k]) a[k] = ld(j[k]);
 #pragma unroll
                 for (int k = 0; k < W; ++k) fn(a[k], j[k], ok[k]);
             }
@@ -542,6 +547,7 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
         ++nxt;
         nx = tab[(nxt < (uint32_t)kFlatEntries ? nxt : 0u) * BS + t];
     };
+    decltype(ld(0u)) a[W] = {};  // as in walk_masks: ok = false loads nothing
     while (m) {
         uint32_t j[W];
         bool ok[W];
@@ -558,9 +564,10 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
                 if (!m) advance();
             }
         }
-        decltype(ld(0u)) a[W];
+        a[0] = ld(j[0]);
 #pragma unroll
-        for (int k = 0; k < W; ++k) a[k] = ld(j[k]);
+        for (int k = 1; k < W; ++k)
+            if (ok[k]) a[k] = ld(j[k]);
 #pragma unroll
         for (int k = 0; k < W; ++k) fn(a[k], j[k], ok[k]);
     }
@@ -569,7 +576,11 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
 
 // Walk over the neighbour slots of lane i: with flat set, the flattened walk
 // when masks are in use and every lane's rows fit it, else walk_masks.  The
-// choice is wave-uniform and does not change the result.
+// choice is wave-uniform and does not change the result.  A callback is
+// inlined into both walks (and every block size), and an implicit
+// multiply-add contraction may differ between those copies: the callbacks
+// write their multiply-adds as explicit fmaf, so every copy computes the
+// same bits.
 template <int BS, int W, int WR, class Ld, class Fn>
 __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, bool use, int flat, uint32_t nslots, uint32_t i,
@@ -643,14 +654,14 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
             if (FULL && r != 4) {
                 const float t = fmaxf(c.h2 - rr, 0.0f);
-                S += t * t * t;
+                S = __builtin_fmaf(t * t, t, S);
                 cur = shift_in(cur, lanes(rr <= h2s));
             } else {
                 uint64_t live = lanes(k < len);
                 if (r == 4) live &= ~lanes(k == kself);
                 const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
                 const float t = keep(c.h2 - rr, in);
-                S += t * t * t;
+                S = __builtin_fmaf(t * t, t, S);
                 cur = shift_in(cur, ins);
             }
         };
@@ -693,7 +704,7 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             const float rr = dist2(pi, at32(Ps, bb + k));
             const bool in = rr <= c.h2 && (r != 4 || k != kself);
             const float t = in ? c.h2 - rr : 0.0f;
-            S += t * t * t;
+            S = __builtin_fmaf(t * t, t, S);
         }
         const size_t at = (size_t)r * nslots + i;
         s0 = n0 ? __brev(s0) >> (32u - n0) : 0u;  // n0 == 0: nothing scanned
@@ -720,14 +731,14 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
                     const float4 q = at32(Ps, j);
                     if (!(dist2(pi, q) <= c.h2)) continue;
                     const double t = fmax(c.hh_d - dist2_d(pi, q), 0.0);
-                    Sd += t * t * t;
+                    Sd = __builtin_fma(t * t, t, Sd);
                 }
             } else {
                 for (uint32_t j = b[r]; j < e[r]; ++j) {
                     const float4 q = at32(Ps, j);
                     if (j == i || !(dist2(pi, q) <= c.h2)) continue;
                     const double t = fmax(c.hh_d - dist2_d(pi, q), 0.0);
-                    Sd += t * t * t;
+                    Sd = __builtin_fma(t * t, t, Sd);
                 }
             }
         }
@@ -780,9 +791,9 @@ __device__ __forceinline__ void normal_term(const Consts &c, float4 pi, float4 q
             t = (float)fmax(c.hh_d - dist2_d(pi, q), 0.0);
     }
     const float w = t * t * fmaxf(q.w, 0.0f);
-    sx += w * (pi.x - q.x);
-    sy += w * (pi.y - q.y);
-    sz += w * (pi.z - q.z);
+    sx = __builtin_fmaf(w, pi.x - q.x, sx);
+    sy = __builtin_fmaf(w, pi.y - q.y, sy);
+    sz = __builtin_fmaf(w, pi.z - q.z, sz);
 }
 
 template <int BS>
@@ -928,38 +939,39 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
                         // and its pressure term in FP64: 1/rho^2 of a density
                         // near FLT_MIN overflows FP32 (an infinite force).
                         const double cmd = (double)c.mCgs * hrd * hrd * (double)inv_r;
-                        const double midd = (double)pj * ((double)iri * (double)iri) +
-                                            (double)p_i * ((double)irj * (double)irj);
+                        const double midd = __builtin_fma((double)pj, (double)iri * (double)iri,
+                                                          (double)p_i * ((double)irj * (double)irj));
                         const double cprd = cmd * midd;
-                        Pdx += cprd * (double)rx;
-                        Pdy += cprd * (double)ry;
-                        Pdz += cprd * (double)rz;
+                        Pdx = __builtin_fma(cprd, (double)rx, Pdx);
+                        Pdy = __builtin_fma(cprd, (double)ry, Pdy);
+                        Pdz = __builtin_fma(cprd, (double)rz, Pdz);
                     } else {
-                        const float mid = pj * iri2 + p_i * (irj * irj);
+                        const float mid = __builtin_fmaf(pj, iri2, p_i * (irj * irj));
                         const float cpr = cm * mid;
-                        Px += cpr * rx;
-                        Py += cpr * ry;
-                        Pz += cpr * rz;
+                        Px = __builtin_fmaf(cpr, rx, Px);
+                        Py = __builtin_fmaf(cpr, ry, Py);
+                        Pz = __builtin_fmaf(cpr, rz, Pz);
                     }
                     if constexpr (V) {
                         // Approaching pairs only (ux < 0); the others add exact zeros.
                         const float4 vj = cj.v, dj = cj.d;
-                        const float ux = -((vi.x - vj.x) * rx + (vi.y - vj.y) * ry + (vi.z - vj.z) * rz);
+                        const float ux =
+                            -__builtin_fmaf(vi.z - vj.z, rz, __builtin_fmaf(vi.y - vj.y, ry, (vi.x - vj.x) * rx));
                         const float mu = c.h * ux * frcp(rr + c.visc_eps);
                         const float Pi = -c.nu * (cs_i + dj.z) * mu * frcp(rho_i + dj.x);
                         const float cv = keep(cm * Pi, lanes(ux < 0.0f));
-                        Qx += cv * rx;
-                        Qy += cv * ry;
-                        Qz += cv * rz;
+                        Qx = __builtin_fmaf(cv, rx, Qx);
+                        Qy = __builtin_fmaf(cv, ry, Qy);
+                        Qz = __builtin_fmaf(cv, rz, Qz);
                     }
                     if constexpr (T) {
                         const float4 nj = cj.n;
                         const float hr3r3 = hr * hr * hr * r * r * r;
                         const float coh = (r > c.halfh) ? hr3r3 : 2.0f * hr3r3 - c.coh_off;
                         const float ch = -c.m2 * coh * inv_r;
-                        Tx += ch * rx - c.sigma_m * (ni.x - nj.x);
-                        Ty += ch * ry - c.sigma_m * (ni.y - nj.y);
-                        Tz += ch * rz - c.sigma_m * (ni.z - nj.z);
+                        Tx += __builtin_fmaf(ch, rx, -c.sigma_m * (ni.x - nj.x));
+                        Ty += __builtin_fmaf(ch, ry, -c.sigma_m * (ni.y - nj.y));
+                        Tz += __builtin_fmaf(ch, rz, -c.sigma_m * (ni.z - nj.z));
                     }
                 });
         };
@@ -1133,13 +1145,13 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
                         const double dx = ((double)q.x + (double)lj.x) - ((double)xi.x + (double)li.x);
                         const double dy = ((double)q.y + (double)lj.y) - ((double)xi.y + (double)li.y);
                         const double dz = ((double)q.z + (double)lj.z) - ((double)xi.z + (double)li.z);
-                        t = keep((float)fmax(c.hh_d - (dx * dx + dy * dy + dz * dz), 0.0), in);
+                        t = keep((float)fmax(c.hh_d - __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)), 0.0), in);
                     }
                 }
                 const float w = keep(t * t * t * frcp(sr), in);
-                sx += w * (vj.x - vi.x);
-                sy += w * (vj.y - vi.y);
-                sz += w * (vj.z - vi.z);
+                sx = __builtin_fmaf(w, vj.x - vi.x, sx);
+                sy = __builtin_fmaf(w, vj.y - vi.y, sy);
+                sz = __builtin_fmaf(w, vj.z - vi.z, sz);
             });
     };
     if (__builtin_expect(flag_set(fl, kFragileByte), 0) && lanes(rows_near(near, b, e)))
