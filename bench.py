@@ -37,7 +37,9 @@ served: a seeded Poisson stream of jobs (4k-256k particles, 200-2000 steps,
 own box and gravity each) through the continuous-batching server under an
 HBM budget (--serve-budget-gb).  The same jobs all submitted at once
 (saturated) measure the server's capacity under the budget; the Poisson
-stream is then offered 0.9 of it.  Reported: aggregate particle-steps/s,
+stream's rate is then calibrated (one run) so that it holds the pool about
+80% occupied.  Reported: aggregate particle-steps/s over the whole run and
+over its steady window (first eighth of the arrivals to the last arrival),
 p50/p99 per-step and per-job latency, mean occupancy, and the rate of the
 same jobs stepped alone on a sample of them.
 
@@ -188,19 +190,47 @@ def served(args) -> dict:
     srv.close()
     sat_value = st_sat["particle_steps"] / r_sat["wall_s"]
     sat_lat = np.array(list(r_sat["latency_ms"].values()))
-    # The Poisson stream, at the rate that keeps the pool 80% occupied:
-    # throughput grows with the resident particles, so at occupancy 0.8 the
-    # server delivers 0.8 / (saturated occupancy) of its saturated rate.
+    # The Poisson stream, at the rate that keeps the pool 80% occupied.  The
+    # first run takes the rate from the saturated one (throughput taken as
+    # proportional to the resident particles: 0.8 / (saturated occupancy) of
+    # the saturated rate); per resident particle the server runs faster when
+    # the pool is less full, so that run settles below 0.8.  Occupancy rises
+    # steeply near capacity (queueing), so the rate is then searched by
+    # interpolation between the nearest runs below and above 0.8, up to four
+    # runs in all, stopping within 0.05 of it; the run closest to 0.8 is the
+    # one reported.
     target_occ = 0.8
-    rate = sat_value * (target_occ / max(st_sat["mean_occupancy"], 1e-9)) / work
-    jobs = at_rate(rate)
-    log(f"[bench] serve: {len(jobs)} jobs, rate {rate:.2f}/s, budget {args.serve_budget_gb} GB = {cap} slots")
-    srv = Server.for_budget(budget, cells_per_particle=1.5, max_scenes=1024)
-    r = drive(srv, jobs, log=log)
-    st = srv.stats()
-    srv.close()
+
+    def stream(rate: float):
+        jobs = at_rate(rate)
+        log(f"[bench] serve: {len(jobs)} jobs, rate {rate:.2f}/s, budget {args.serve_budget_gb} GB = {cap} slots")
+        srv = Server.for_budget(budget, cells_per_particle=1.5, max_scenes=1024)
+        r = drive(srv, jobs, log=log)
+        st = srv.stats()
+        srv.close()
+        steady = steady_window(r, cap, jobs)
+        occ = steady["mean_occupancy"] if steady["mean_occupancy"] is not None else st["mean_occupancy"]
+        log(f"[bench] serve: rate {rate:.2f}/s held the pool {occ:.2f} occupied")
+        return {"rate": rate, "occ": occ, "jobs": jobs, "r": r, "st": st, "steady": steady}
+
+    runs = [stream(sat_value * (target_occ / max(st_sat["mean_occupancy"], 1e-9)) / work)]
+    runs.append(stream(runs[0]["rate"] * target_occ / max(runs[0]["occ"], 1e-3)))
+    while len(runs) < 4 and min(abs(x["occ"] - target_occ) for x in runs) > 0.05:
+        below = [x for x in runs if x["occ"] < target_occ]
+        above = [x for x in runs if x["occ"] >= target_occ]
+        if not below or not above:
+            x = runs[-1]
+            runs.append(stream(x["rate"] * target_occ / max(x["occ"], 1e-3)))
+            continue
+        lo = max(below, key=lambda x: x["occ"])
+        hi = min(above, key=lambda x: x["occ"])
+        runs.append(stream(lo["rate"] + (hi["rate"] - lo["rate"]) * (target_occ - lo["occ"]) /
+                           max(hi["occ"] - lo["occ"], 1e-3)))
+    best = min(runs, key=lambda x: abs(x["occ"] - target_occ))
+    calibration = [{"rate": x["rate"], "steady_occupancy": x["occ"], "steady_value": x["steady"]["value"]}
+                   for x in runs]
+    rate, jobs, r, st, steady = best["rate"], best["jobs"], best["r"], best["st"], best["steady"]
     value = st["particle_steps"] / r["wall_s"]
-    steady = steady_window(r, cap, jobs)
     lat = np.array(list(r["latency_ms"].values()))
     # The same jobs alone, on a sample of 32: each stepped for its full step
     # count (the same work it does in the server), one after another, with
@@ -230,12 +260,13 @@ def served(args) -> dict:
         f"({sat_value / seq:.2f}x alone), occupancy {st_sat['mean_occupancy']:.2f}, "
         f"step p50/p99 {st_sat['step_p50_ms']:.2f}/{st_sat['step_p99_ms']:.2f} ms")
     return {
-        "workload": f"{len(jobs)} jobs, Poisson rate {rate:.3f}/s (tuned for {target_occ:.0%} occupancy of the "
-                    f"pool), particles log-uniform in "
+        "workload": f"{len(jobs)} jobs, Poisson rate {rate:.3f}/s (calibrated for {target_occ:.0%} occupancy of "
+                    f"the pool), particles log-uniform in "
                     f"[4096, 262144], steps uniform in [200, 2000], own box and gravity each, dt=1e-4",
         "budget_bytes": budget, "budget_note": f"{args.serve_budget_gb} GB device budget (the 200 GB of the "
                     f"workload's statement would need hours of arrivals to fill; the admission logic is the same)",
         "slot_cap": cap, "jobs": len(jobs),
+        "calibration": calibration,
         "value": value, "unit": "particle-steps/s",
         "steady": {"window_s": [steady["t0_s"], steady["t1_s"]], "value": steady["value"],
                    "mean_occupancy": steady["mean_occupancy"],
