@@ -605,6 +605,28 @@ def test_neighbour_masks_are_bitwise_transparent(gpu, case):
             assert np.array_equal(outs[0][k], outs[m][k]), f"{case}: {k} differs between {name} and full scans"
 
 
+@pytest.mark.parametrize("case", ["c2", "c3"])
+def test_neighbour_masks_are_bitwise_transparent_at_full_size(gpu, case):
+    """The walks over the density pass's masks against the full stencil
+    scan at full size: the 1,048,576-particle column and the 8,388,608-
+    particle block-and-spray scene (dense cells and near-isolated spray),
+    5 substeps each, every bit of the final state."""
+    cfg = {"c2": configs.column_c2, "c3": configs.spray_c3}[case]()
+    outs = []
+    for scan in (True, False):
+        sc = gpu.Scene(cfg.params, cfg.n)
+        try:
+            sc.upload(cfg.pos, cfg.vel)
+            sc.set_debug(forces=False, scan=scan)
+            sc.run(cfg.dt, 5)
+            sc.sync()
+            outs.append(sc.download(L.LF_ORDER_ID))
+        finally:
+            sc.close()
+    for k in ("pos", "vel", "acc", "colour"):
+        assert np.array_equal(outs[0][k], outs[1][k]), f"{case}: {k} differs between mask walks and full scans"
+
+
 def test_lazy_normals_match_per_substep_pass(gpu):
     """Without surface tension the normals are skipped while stepping and
     computed when read; the trajectory is bitwise that with the normals pass
