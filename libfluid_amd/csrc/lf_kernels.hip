@@ -426,9 +426,10 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
     }
 }
 
-// The density pass records, per row, a mask of the row's slots that are
-// neighbours (bit k = slot b[r] + k), once for radius h and once for the
-// skin radius.  walk_masks visits the set bits in increasing slot order, the
+// The density pass records, per row, a mask of the row's slots within the
+// skin radius (bit k = slot b[r] + k), a superset of the neighbours within h,
+// which the callbacks re-test.  walk_masks visits the set bits in increasing
+// slot order, the
 // same order as a full scan; a row longer than 64 slots has no mask and is
 // scanned in full.  With use == false every row is scanned.  Each step takes
 // up to W set bits of the row and issues all their candidates' loads (ld)
