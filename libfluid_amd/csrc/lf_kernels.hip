@@ -439,7 +439,7 @@ __device__ __forceinline__ void rows9(const Consts &c, const uint32_t *__restric
 // that candidate, whatever finite values the previous step loaded there);
 // calling it unconditionally keeps the later candidates' loads ahead of the
 // first candidate's arithmetic.
-template <int W, class Ld, class Fn>
+template <int W, bool PAIR = false, class Ld, class Fn>
 __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e, const RowMasks &mask, bool use,
                                            uint32_t nslots, uint32_t i, Ld &&ld, Fn &&fn) {
     uint64_t mk[9];
@@ -453,6 +453,45 @@ __device__ __forceinline__ void walk_masks(const uint32_t *b, const uint32_t *e,
         mk[r] = 0ull;
         if (use && len <= 32u) mk[r] = mask.w32[at];
         else if (use && len <= 64u) mk[r] = mask.w64[at];
+    }
+    if constexpr (PAIR) {
+        // Where every row of the wave fits 32 slots, rows are walked in
+        // pairs: row 2g's mask in the low and row 2g + 1's in the high word
+        // of one 64-bit stream, so a step's padding is paid per pair, not
+        // per row.  Bits stay in slot order (the pair's first row first).
+        bool lng = !use;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) lng = lng || e[r] - b[r] > 32u;
+        if (!__ballot(lng)) {
+#pragma unroll
+            for (int g = 0; g < 5; ++g) {
+                const int r0 = 2 * g, r1 = 2 * g + 1;
+                uint64_t m = mk[r0] | (r1 < 9 ? mk[r1] << 32 : 0ull);
+                const uint32_t b0 = b[r0], b1 = r1 < 9 ? b[r1] - 32u : 0u;  // slot = (bit < 32 ? b0 : b1) + bit
+                while (m) {
+                    uint32_t j[W];
+                    bool ok[W];
+                    uint32_t k0 = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    j[0] = (k0 < 32u ? b0 : b1) + k0;
+                    ok[0] = true;
+                    m &= m - 1ull;
+#pragma unroll
+                    for (int k = 1; k < W; ++k) {
+                        ok[k] = m != 0ull;
+                        const uint32_t kk = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                        j[k] = ok[k] ? (kk < 32u ? b0 : b1) + kk : j[0];
+                        m &= m - 1ull;  // stays 0 when it was 0
+                    }
+                    a[0] = ld(j[0]);
+#pragma unroll
+                    for (int k = 1; k < W; ++k)
+                        if (ok[k]) a[k] = ld(j[k]);
+#pragma unroll
+                    for (int k = 0; k < W; ++k) fn(a[k], j[k], ok[k]);
+                }
+            }
+            return;
+        }
     }
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
@@ -582,12 +621,12 @@ __device__ __forceinline__ bool walk_flat(uint2 *__restrict__ tab, const uint32_
 // multiply-add contraction may differ between those copies: the callbacks
 // write their multiply-adds as explicit fmaf, so every copy computes the
 // same bits.
-template <int BS, int W, int WR, class Ld, class Fn>
+template <int BS, int W, int WR, bool PAIR = false, class Ld, class Fn>
 __device__ __forceinline__ void walk_rows(uint2 *__restrict__ tab, const uint32_t *b, const uint32_t *e,
                                           const RowMasks &mask, bool use, int flat, uint32_t nslots, uint32_t i,
                                           Ld &&ld, Fn &&fn) {
     if (use && flat && walk_flat<BS, W>(tab, b, e, mask, nslots, i, flat == 2, ld, fn)) return;
-    walk_masks<WR>(b, e, mask, use, nslots, i, ld, fn);
+    walk_masks<WR, PAIR>(b, e, mask, use, nslots, i, ld, fn);
 }
 
 // 5. density + Tait EOS (computeDensity, fluidphysics.cu:8-84); also the
@@ -1127,7 +1166,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     };
     __shared__ uint2 tab[kFlatEntries * BS];
     auto walk = [&]<bool F>() {
-        walk_rows<BS, F ? 2 : kXsphWalkWidth, F ? 2 : kXsphRowWidth>(
+        walk_rows<BS, F ? 2 : kXsphWalkWidth, F ? 2 : kXsphRowWidth, !F>(
             tab, b, e, MS, masks_ok, flat, nslots, i,
             [&](uint32_t j) { return Cand{at32(Pn, j), at32(Vh, j)}; },
             [&](const Cand &cj, uint32_t j, bool ok) {
