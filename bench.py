@@ -68,6 +68,15 @@ HBM_PEAK = 8.0e12
 PASS_BYTES = {"density": 28, "normals": 36, "forces": 124, "xsph": 52}
 PASS_NAMES = {"density": "k_density (density + Tait EOS + neighbour masks)", "normals": "k_normals",
               "forces": "k_forces (forces + leapfrog integrator)", "xsph": "k_xsph (XSPH)"}
+# What bounds each pass on chip (pipe counters of the column scene: VALUBusy and texture-unit busy
+# shares, profiles/pipes_column_c2.md); HBM is far from the limit for all three.
+PASS_LIMITER = {
+    "density": "VALU issue (87% busy) and the texture path (79% busy): 216 candidate slots per particle at "
+               "12 VALU instructions each; ceiling about 4% of HBM",
+    "normals": "texture path (scattered 16-byte gathers)",
+    "forces": "texture path of the neighbour gathers (texture data unit 76% busy, 33% waiting on L2)",
+    "xsph": "texture path of the neighbour gathers (texture data unit 78% busy)",
+}
 STEP_BYTES_PER_PARTICLE = 436   # compulsory bytes of the whole substep, plus 12 B per cell
 NORMALS_BYTES = 36              # of which the normals pass, which plain scenes skip
 REF_PARTICLE_STEPS = 7.7e7      # README.md:2 of the reference: 1M particles at 77 FPS (RTX 2080), 1 substep/frame
@@ -629,6 +638,7 @@ def main():
             "traffic": traffic,
             "traffic_source": tsrc,
             "kernel": PASS_NAMES[dom],
+            "limiter": PASS_LIMITER[dom],
             "bytes_per_particle": PASS_BYTES[dom],
             "avg_ms": neigh[dom]["avg_ms"],
             "launches": neigh[dom]["launches"],
