@@ -71,7 +71,7 @@ PASS_NAMES = {"density": "k_density (density + Tait EOS + neighbour masks)", "no
 # What bounds each pass on chip (pipe counters of the column scene: VALUBusy and texture-unit busy
 # shares, profiles/pipes_column_c2.md); HBM is far from the limit for all three.
 PASS_LIMITER = {
-    "density": "VALU issue (87% busy) and the texture path (79% busy): 216 candidate slots per particle at "
+    "density": "VALU issue (87% busy) and the texture path (79% busy): 216 candidate slots per particle on the column, at "
                "12 VALU instructions each; ceiling about 4% of HBM",
     "normals": "texture path (scattered 16-byte gathers)",
     "forces": "texture path of the neighbour gathers (texture data unit 76% busy, 33% waiting on L2)",
