@@ -53,8 +53,10 @@ struct Consts {
     // and XSPH passes walk masks at all (off: every row is scanned)
     float skin_q;
     int use_masks;
-    // density + EOS
-    float mC6;  // m * 315/(64 pi h^9)
+    // density + EOS: the candidate loop sums t^3 with t = (h2 - r^2) / h2,
+    // so a pair's m poly6(r) is mC6h6 t^3
+    float inv_h2;  // 1 / h2
+    float mC6h6;   // m * 315/(64 pi h^9) * h2^3
     float rho0, inv_rho0, eos_scale, gamma, gamma_m1, K;
     // gamma and gamma - 1 when they are integers in [1, 15], else 0
     // (tait_pow then multiplies instead of calling powf)
@@ -75,7 +77,8 @@ struct Consts {
     // far out) turn the FP32 cancellation in h^2 - r^2 as r -> h into large
     // relative errors of huge values (1/rho^2).  Pairs touching one take the
     // pair geometry in FP64: hh_d = h^2, mC6_d = m * C6; fragile_S bounds the
-    // FP32 sum of (h^2 - r^2)^3 below which the density is recomputed;
+    // FP32 sum of t^3 (t = (h2 - r^2) / h2) below which the density is
+    // recomputed;
     // fragile_ir is the 1/rho above which a particle is near-isolated,
     // fragile_sr the rho_i + rho_j below which an XSPH pair is.
     double hh_d, mC6_d;

@@ -661,19 +661,26 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     uint32_t b[9], e[9];
     rows9(c, st, shift, pi, b, e);
     float S = 0.0f;
-    // Branch-free candidate tests: a candidate outside h (or the particle
-    // itself, or past the end of the lane's row) adds an exact zero to S;
-    // in a whole group of four the term is max(h^2 - r^2, 0), the same value.
+    // Branch-free candidate tests.  A candidate's r^2 is formed with the
+    // skin radius folded in, rs = r^2 - h_s^2 (the multiply-add chain starts
+    // from -h_s^2): its sign bit is the skin-mask bit, and its density term
+    // is t^3 with t = (h^2 - r^2) / h^2 = c1 - rs / h^2, one multiply-add
+    // whose result is clamped to [0, 1] by the instruction itself (t lies in
+    // [0, 1] for every r).  A candidate outside h, or past the end of the
+    // lane's row, adds an exact zero to S; the particle itself (centre row)
+    // has its rs replaced by 1, which gives t = 0 and a clear bit.
     // The first 64 slots of every lane's row are scanned four slots per step
     // (slots 0..31 into one mask word, 32..63 into another): the sorted
     // positions are also kept as x, y and z planes, so one 16-byte load per
     // plane brings four candidates.  The slot offset k is a scalar, so a
     // load's address is the lane's row offset plus a scalar (buffer
-    // soffset).  Each mask word is built as m = 2m + bit, one add-with-carry
-    // per candidate, which leaves slot k at bit n-1-k of the n slots scanned
-    // into it; one bit reversal per row moves it to bit k.  Slots past 64
-    // (rows without masks) are tested per lane.  Only the centre row (r = 4:
-    // dz = dy = 0) holds the particle itself.
+    // soffset).  Each mask word is built as m = 2m + bit (a funnel shift of
+    // the sign bit in), which leaves slot k at bit n-1-k of the n slots
+    // scanned into it; one bit reversal per row moves it to bit k.  Slots
+    // past 64 (rows without masks) are tested per lane.  Only the centre row
+    // (r = 4: dz = dy = 0) holds the particle itself.
+    const float nh2s = -h2s, ninv_h2 = -c.inv_h2;
+    const float c1 = (c.h2 - h2s) * c.inv_h2;  // h2 - h2s is exact: h2 <= h2s <= 2 h2
     const __amdgpu_buffer_rsrc_t px = plane_buffer(soa, nslots), py = plane_buffer(soa + nslots, nslots),
                                  pz = plane_buffer(soa + 2 * (size_t)nslots, nslots);
 #pragma unroll
@@ -691,18 +698,18 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         const uint32_t lm = len < 64u ? len : 64u, nf = lm & ~3u;
         uint32_t cur = 0u, s0 = 0u;  // the mask word being built; slots 0..31 once past them
         auto test = [&]<bool FULL>(uint32_t k, float qx, float qy, float qz) {
-            const float rr = dist2(pi, make_float4(qx, qy, qz, 0.0f));
-            if (FULL && r != 4) {
-                const float t = fmaxf(c.h2 - rr, 0.0f);
+            const float dx = qx - pi.x, dy = qy - pi.y, dz = qz - pi.z;
+            float rs = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, nh2s)));
+            if (r == 4) rs = (k == kself) ? 1.0f : rs;
+            float t = fminf(fmaxf(__builtin_fmaf(rs, ninv_h2, c1), 0.0f), 1.0f);
+            if (FULL) {
                 S = __builtin_fmaf(t * t, t, S);
-                cur = shift_in(cur, lanes(rr <= h2s));
+                cur = __builtin_amdgcn_alignbit(cur, __float_as_uint(rs), 31u);
             } else {
-                uint64_t live = lanes(k < len);
-                if (r == 4) live &= ~lanes(k == kself);
-                const uint64_t in = lanes(rr <= c.h2) & live, ins = lanes(rr <= h2s) & live;
-                const float t = keep(c.h2 - rr, in);
+                const uint64_t live = lanes(k < len);
+                t = keep(t, live);
                 S = __builtin_fmaf(t * t, t, S);
-                cur = shift_in(cur, ins);
+                cur = shift_in(cur, lanes(rs < 0.0f) & live);
             }
         };
         auto group = [&]<bool FULL>(float4 X, float4 Y, float4 Z, uint32_t k0) {
@@ -741,10 +748,12 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         else
             s0 = cur;
         for (k = 64u; k < len; ++k) {  // rows of more than 64 slots: no masks
-            const float rr = dist2(pi, at32(Ps, bb + k));
-            const bool in = rr <= c.h2 && (r != 4 || k != kself);
-            const float t = in ? c.h2 - rr : 0.0f;
-            S = __builtin_fmaf(t * t, t, S);
+            const float4 q = at32(Ps, bb + k);
+            const float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+            const float rs = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, nh2s)));
+            const float t = fminf(fmaxf(__builtin_fmaf(rs, ninv_h2, c1), 0.0f), 1.0f);
+            const float tk = (r != 4 || k != kself) ? t : 0.0f;
+            S = __builtin_fmaf(tk * tk, tk, S);
         }
         const size_t at = (size_t)r * nslots + i;
         s0 = n0 ? __brev(s0) >> (32u - n0) : 0u;  // n0 == 0: nothing scanned
@@ -784,7 +793,7 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         }
         S = (float)(Sd * c.mC6_d);
     } else {
-        S *= c.mC6;  // sum of m * poly6
+        S *= c.mC6h6;  // sum of m * poly6
     }
     // Boundary density term rho0 * (1 - boundaryDensity(d)) for d = |sdf| < h
     // (fluidphysics.cu:55-61, fluidkernel.cu:45-67), in FP64, written in t = d/h.

@@ -92,7 +92,9 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.eps = FLT_MIN;
     c.skin_q = (float)(2.2 * 4096.0 / h);  // the substep's skin is sized on the device from its largest move
     c.use_masks = (debug_flags & LF_DEBUG_SCAN) ? 0 : 1;
-    c.mC6 = (float)(m * C6);
+    const double h2f = (double)c.h2, h6f = h2f * h2f * h2f;
+    c.inv_h2 = (float)(1.0 / h2f);
+    c.mC6h6 = (float)(m * C6 * h6f);
     c.rho0 = p.rest_density;
     const double inv_rho0 = p.rest_density == 0.0f ? 1.0 : 1.0 / (double)p.rest_density;
     const double inv_gamma = p.gamma == 0.0f ? 1.0 : 1.0 / (double)p.gamma;
@@ -109,7 +111,7 @@ Consts make_consts(const lf_params &p, float dt, int debug_flags) {
     c.hh_d = h * h;
     c.mC6_d = m * C6;
     const double fragile_rho = (double)p.rest_density / 64.0;
-    c.fragile_S = m * C6 > 0.0 ? (float)(fragile_rho / (m * C6)) : 0.0f;
+    c.fragile_S = m * C6 > 0.0 ? (float)(fragile_rho / (m * C6 * h6f)) : 0.0f;
     c.fragile_ir = (float)(1.0 / fragile_rho);
     c.fragile_sr = (float)(2.0 * fragile_rho);
     c.hmCg6 = (float)(h * m * Cg6);
