@@ -71,8 +71,8 @@ PASS_NAMES = {"density": "k_density (density + Tait EOS + neighbour masks)", "no
 # What bounds each pass on chip (pipe counters of the column scene: VALUBusy and texture-unit busy
 # shares, profiles/pipes_column_c2.md); HBM is far from the limit for all three.
 PASS_LIMITER = {
-    "density": "VALU issue (87% busy) and the texture path (79% busy): 216 candidate slots per particle on the column, at "
-               "12 VALU instructions each; ceiling about 4% of HBM",
+    "density": "candidate work: the texture path (77-79% busy) and VALU issue (77% busy), 216 candidate slots per "
+               "particle on the column at 10 VALU instructions and 12 loaded bytes each; ceiling about 4% of HBM",
     "normals": "texture path (scattered 16-byte gathers)",
     "forces": "texture path of the neighbour gathers (texture data unit 76% busy, 33% waiting on L2)",
     "xsph": "texture path of the neighbour gathers (texture data unit 78% busy)",
