@@ -72,6 +72,31 @@ def test_batch_matches_solo_scenes(gpu):
             assert_same(b.download(i), ref[j], f"scene {j}")
 
 
+@pytest.mark.parametrize("graphs", [True, False])
+def test_ragged_batch_with_tiny_scenes(gpu, graphs):
+    """Scenes of 1, 2, 511, 512 and 513 particles (no neighbours at all, a
+    chunk filled exactly, one slot into the next chunk) beside a larger one:
+    every batched scene equals its solo run bit for bit, with graph replay
+    and with eager launches."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    sizes = (1, 2, 511, 512, 513, 3000)
+    cfgs = [configs.block_scene(n, rng.uniform(1.5, 2.5, 3), rng.normal(size=3), 1100 + j, f"ragged{j}")
+            for j, n in enumerate(sizes)]
+    steps = 6
+    ref = [solo(gpu, c, steps, graphs) for c in cfgs]
+    with batch_for(gpu, cfgs) as b:
+        b.set_graphs(graphs)
+        ids = [b.add(c.params, c.n, c.dt) for c in cfgs]
+        for i, c in zip(ids, cfgs):
+            b.upload(i, c.pos, c.vel)
+        b.step(steps)
+        b.sync()
+        for j, (i, c) in enumerate(zip(ids, cfgs)):
+            out = b.download(i)
+            assert np.all(np.isfinite(out["pos"])), f"{c.n} particles: non-finite positions"
+            assert_same(out, ref[j], f"scene of {c.n} particles")
+
+
 def test_batch_graphs_and_eager_agree(gpu):
     cfgs = mixed_scenes(4, seed=6)
     outs = []

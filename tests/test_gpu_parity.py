@@ -513,6 +513,40 @@ def test_pair_one_ulp_inside_the_kernel_edge(gpu, oracle):
     assert abs(float(np.linalg.norm(g["acc"][0])) - float(cfg.params.acc_limit)) < 1e-3
 
 
+def test_single_particle_and_empty_scene(gpu, oracle):
+    """A scene of one particle has no neighbour: its density is rho0 (the
+    stencil sum is zero) and, away from the walls, its pressure is zero;
+    within h of a wall the boundary term alone sets its pressure while its
+    density stays rho0.  It falls freely (a = g) and every field matches the
+    oracle: nonzero ones within TOL, zero ones exactly.  A scene of no
+    particles is refused with LF_ERR_INVALID (the reference launches empty
+    kernels and steps nothing)."""
+    cfg = configs.dambreak_c1()
+    h = float(np.float32(cfg.params.h))
+    for where, p in (("mid-box", [0.5, 0.25, 0.25]), ("near the z = 0 wall", [0.5, 0.25, 0.4 * h])):
+        pos = np.array([p], np.float32)
+        vel = np.array([[0.01, -0.02, 0.005]], np.float32)
+        acc = np.zeros_like(pos)
+        g = gpu_substep(gpu, cfg, pos, vel, acc)
+        o = oracle_substep(oracle, cfg, pos, vel, acc)
+        assert g["density"][0] == np.float32(cfg.params.rest_density), where
+        for k in ("density", "pressure", "f_pressure", "f_viscosity", "vel_half", "pos", "vel", "acc"):
+            gv = np.atleast_1d(np.asarray(g[k][0], np.float64))
+            ov = np.atleast_1d(np.asarray(o[k][0], np.float64))
+            if not np.any(ov):
+                assert not np.any(gv), f"{where}: {k} = {gv}, expected zero"
+            else:
+                rel = np.linalg.norm(gv - ov) / np.linalg.norm(ov)
+                assert rel <= TOL, f"{where}: {k} relative error {rel:.3e}"
+        if where == "mid-box":
+            assert g["pressure"][0] == 0.0
+        else:
+            assert g["pressure"][0] != 0.0, "the boundary term must set the pressure"
+    with pytest.raises(L.LfError) as err:
+        gpu.Scene(cfg.params, 0)
+    assert err.value.code == L.LF_ERR_INVALID
+
+
 def test_spray_only_stays_finite_and_matches_after_150_substeps(gpu, oracle_omp):
     """262,144 points scattered through a box with a few neighbours each:
     near-isolated particles (FP64 pair paths) everywhere.  The state stays
