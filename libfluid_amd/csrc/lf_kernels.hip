@@ -688,8 +688,8 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
         const uint32_t bb = b[r], len = e[r] - b[r];
         const uint32_t off = bb * (uint32_t)sizeof(float);
         const uint32_t kself = i - bb;  // < len only in the centre row
-        // Slots scanned: up to the wave's longest row (at most 64), rounded up
-        // to a multiple of 4; slots past a lane's row add nothing.
+        // Slots scanned: the lane's row (at most 64 slots) rounded up to a
+        // multiple of 4; slots past the row add nothing.
         // Slots with mask bits: [0, lm); whole groups of four: [0, nf).  The
         // group loop's bound is per lane (a lane leaves it when its row's
         // whole groups are done), so a group's four tests need no end-of-row
