@@ -316,3 +316,14 @@ def test_flat_walk_table(case):
         assert cnt == 16
     if case == "seventeen":
         assert cnt == 17  # above the table: the kernels walk such a wave row by row
+
+
+def test_bench_measures_one_gpu_only():
+    """bench.py --gpus N with N > 1 exits at once with a message (the path is
+    measured on one GPU; nothing is sharded), before touching a device."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["python3", os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stdout + r.stderr
+    assert "{" not in r.stdout  # no JSON line
