@@ -731,13 +731,16 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             group.template operator()<true>(AX, AY, AZ, k);
             group.template operator()<true>(BX, BY, BZ, k + 4u);
         }
+        // After the loop the slot offset differs between lanes: it goes into
+        // the per-lane offset (a per-lane scalar offset would be issued once
+        // per distinct value in the wave).
         if (k < nf) {
-            const uint32_t so = k * (uint32_t)sizeof(float);
-            group.template operator()<true>(load4(px, off, so), load4(py, off, so), load4(pz, off, so), k);
+            const uint32_t vo = off + k * (uint32_t)sizeof(float);
+            group.template operator()<true>(load4(px, vo, 0u), load4(py, vo, 0u), load4(pz, vo, 0u), k);
         }
         if (nf < lm) {  // the partial group
-            const uint32_t so = nf * (uint32_t)sizeof(float);
-            group.template operator()<false>(load4(px, off, so), load4(py, off, so), load4(pz, off, so), nf);
+            const uint32_t vo = off + nf * (uint32_t)sizeof(float);
+            group.template operator()<false>(load4(px, vo, 0u), load4(py, vo, 0u), load4(pz, vo, 0u), nf);
         }
         // The lane scanned lw = lm rounded up to 4 slots: n0 = min(lw, 32)
         // of them into the low word, lw - 32 into the high word.
