@@ -359,18 +359,10 @@ __device__ __forceinline__ uint32_t scene_move_q(const uint32_t *__restrict__ mo
 
 // Lane-mask forms of per-lane predicates.  lanes(p) is the wave's mask of
 // lanes where p holds (a compare lowers to a single v_cmp into it); combining
-// masks is scalar work.  shift_in and keep take the mask directly: a per-lane
-// bool built from an and of two compares would otherwise be materialised as
-// 0/1 in a vector register and compared again.
+// masks is scalar work.  keep takes the mask directly: a per-lane bool
+// built from an and of two compares would otherwise be materialised as 0/1
+// in a vector register and compared again.
 __device__ __forceinline__ uint64_t lanes(bool p) { return __builtin_amdgcn_ballot_w64(p); }
-
-// m = 2 m + (this lane's bit of mask), one v_addc_co_u32 with the mask as
-// carry-in.
-__device__ __forceinline__ uint32_t shift_in(uint32_t m, uint64_t mask) {
-    uint64_t carry_out;
-    asm("v_addc_co_u32 %0, %1, %0, %0, %2" : "+v"(m), "=s"(carry_out) : "s"(mask));
-    return m;
-}
 
 // this lane's bit of mask ? v : 0, one v_cndmask_b32.
 __device__ __forceinline__ float keep(float v, uint64_t mask) {

@@ -666,9 +666,10 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
     // from -h_s^2): its sign bit is the skin-mask bit, and its density term
     // is t^3 with t = (h^2 - r^2) / h^2 = c1 - rs / h^2, one multiply-add
     // whose result is clamped to [0, 1] by the instruction itself (t lies in
-    // [0, 1] for every r).  A candidate outside h, or past the end of the
-    // lane's row, adds an exact zero to S; the particle itself (centre row)
-    // has its rs replaced by 1, which gives t = 0 and a clear bit.
+    // [0, 1] for every r).  A candidate outside h adds an exact zero to S;
+    // the particle itself (centre row) and the slots of a partial group past
+    // the end of the lane's row have their rs replaced by 1, which gives
+    // t = 0 and a clear bit.
     // The first 64 slots of every lane's row are scanned four slots per step
     // (slots 0..31 into one mask word, 32..63 into another): the sorted
     // positions are also kept as x, y and z planes, so one 16-byte load per
@@ -701,16 +702,10 @@ __global__ __launch_bounds__(BS) void k_density(const Consts *__restrict__ cs, c
             const float dx = qx - pi.x, dy = qy - pi.y, dz = qz - pi.z;
             float rs = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, nh2s)));
             if (r == 4) rs = (k == kself) ? 1.0f : rs;
-            float t = fminf(fmaxf(__builtin_fmaf(rs, ninv_h2, c1), 0.0f), 1.0f);
-            if (FULL) {
-                S = __builtin_fmaf(t * t, t, S);
-                cur = __builtin_amdgcn_alignbit(cur, __float_as_uint(rs), 31u);
-            } else {
-                const uint64_t live = lanes(k < len);
-                t = keep(t, live);
-                S = __builtin_fmaf(t * t, t, S);
-                cur = shift_in(cur, lanes(rs < 0.0f) & live);
-            }
+            if (!FULL) rs = (k < len) ? rs : 1.0f;  // past the row's end
+            const float t = fminf(fmaxf(__builtin_fmaf(rs, ninv_h2, c1), 0.0f), 1.0f);
+            S = __builtin_fmaf(t * t, t, S);
+            cur = __builtin_amdgcn_alignbit(cur, __float_as_uint(rs), 31u);
         };
         auto group = [&]<bool FULL>(float4 X, float4 Y, float4 Z, uint32_t k0) {
             if (k0 == 32u) {  // the low word is complete
