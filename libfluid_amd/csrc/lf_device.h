@@ -189,11 +189,21 @@ constexpr double kSkinMin = 0.01, kSkinMax = 0.1;
 constexpr uint32_t kNearBlock = 64u;
 __device__ __forceinline__ bool rows_near(const uint8_t *__restrict__ near, const uint32_t b[9],
                                           const uint32_t e[9]) {
+    // A row's first and last blocks (most rows span one or two) are loaded
+    // for all nine rows together; blocks between them only for rows of more
+    // than 64 slots.
     uint32_t any = 0u;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        if (b[r] < e[r]) {
+            const uint32_t k0 = b[r] / kNearBlock, k1 = (e[r] - 1u) / kNearBlock;
+            any |= (uint32_t)near[k0] | (uint32_t)near[k1];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < 9; ++r)
         if (b[r] < e[r])
-            for (uint32_t k = b[r] / kNearBlock; k <= (e[r] - 1u) / kNearBlock; ++k) any |= near[k];
+            for (uint32_t k = b[r] / kNearBlock + 1u; k < (e[r] - 1u) / kNearBlock; ++k) any |= near[k];
     return any != 0u;
 }
 // A particle's move in the substep, v dt + a dt^2 / 2, is bounded without
