@@ -158,8 +158,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(uint32_t *__restrict__ cou
     if (prefixed) {
         prefix = bsum[blockIdx.x];
     } else {
-        uint32_t before = 0;
-        for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kBlock) before += bsum[t];
+        uint32_t before = 0, t = threadIdx.x;
+        for (; t + 3u * kBlock < blockIdx.x; t += 4u * kBlock)  // four loads in flight
+            before += bsum[t] + bsum[t + kBlock] + bsum[t + 2u * kBlock] + bsum[t + 3u * kBlock];
+        for (; t < blockIdx.x; t += kBlock) before += bsum[t];
         (void)block_excl_scan(before, sh, prefix);
     }
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanPerThread;
@@ -375,8 +377,13 @@ __global__ __launch_bounds__(kBlock) void k_gather(const Consts *__restrict__ cs
         const uint32_t lo = start[b] + shift, hi = start[b + 1] + shift;
         uint32_t f = s;
         if (hi - lo <= kRankDirectMax) {
-            uint32_t r = 0;
-            for (uint32_t t = lo; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
+            // four members' ids loaded together per step
+            uint32_t r = 0, t = lo;
+            for (; t + 4u <= hi; t += 4u) {
+                const uint32_t a0 = slot_id[t], a1 = slot_id[t + 1u], a2 = slot_id[t + 2u], a3 = slot_id[t + 3u];
+                r += (a0 < id ? 1u : 0u) + (a1 < id ? 1u : 0u) + (a2 < id ? 1u : 0u) + (a3 < id ? 1u : 0u);
+            }
+            for (; t < hi; ++t) r += (slot_id[t] < id) ? 1u : 0u;
             f = lo + r;
         }
         const float4 p = P[src], v = V[src];
