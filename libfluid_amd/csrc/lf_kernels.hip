@@ -46,6 +46,9 @@ namespace lf {
 // inside its bucket.  Particles arrive in the previous cell order, so runs of
 // lanes share a bucket: each run issues one atomic for its length and hands
 // out consecutive ranks.  Counts are zeroed by the previous substep's scan.
+// Each thread takes kHashItems particles, kBlock slots apart, whose loads
+// and count atomics are issued together (the pass is latency bound).
+constexpr int kHashItems = 2;
 __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                                  const float4 *__restrict__ P, uint32_t *__restrict__ keys,
                                                  uint32_t *__restrict__ ranks, uint32_t *__restrict__ counts,
@@ -53,28 +56,45 @@ __global__ __launch_bounds__(kBlock) void k_hash(const Consts *__restrict__ cs, 
                                                  uint32_t *__restrict__ nsegs) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *nsegs = 0u;  // large buckets of the substep, listed by k_scatter
     uint32_t slot0;
-    const Consts *cp = block_scene<kBlock>(cs, chunk, slot0);
+    const Consts *cp = block_scene<kBlock * kHashItems>(cs, chunk, slot0);
     if (!cp) return;
     const Consts &c = *cp;
-    const uint32_t i = slot0 + threadIdx.x;
-    if (i == c.first) set_flag(flags, c.scene, kFragileByte, 0u);  // set by k_density
-    if (i - c.first < kMoveSlots) *move_slot(moves, c.scene, i - c.first) = 0u;  // raised by k_gather
     const int lane = threadIdx.x & 63;
-    const bool active = i < c.first + c.count;
-    const uint32_t b = active ? c.cell_base + bucket_of(c, P[i]) : 0xFFFFFFFFu;
-    const uint32_t prev = __shfl_up(b, 1, 64);
-    const bool head = lane == 0 || prev != b;
-    const uint64_t heads = __ballot(head);
-    const uint64_t upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-    const int first = 63 - __clzll(heads & upto);
-    const uint64_t above = heads & ~upto;
-    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
-    uint32_t base = 0;
-    if (active && lane == first) base = atomicAdd(&counts[b], (uint32_t)(next - first));
-    base = __shfl(base, first, 64);
-    if (active) {
-        keys[i] = b;
-        ranks[i] = base + (uint32_t)(lane - first);
+    uint32_t i[kHashItems], b[kHashItems], base[kHashItems];
+    int first[kHashItems], next[kHashItems];
+    bool active[kHashItems];
+    float4 p[kHashItems];
+#pragma unroll
+    for (int q = 0; q < kHashItems; ++q) {
+        i[q] = slot0 + (uint32_t)(q * kBlock) + threadIdx.x;
+        active[q] = i[q] < c.first + c.count;
+        p[q] = active[q] ? P[i[q]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < kHashItems; ++q) {
+        if (i[q] == c.first) set_flag(flags, c.scene, kFragileByte, 0u);  // set by k_density
+        if (i[q] - c.first < kMoveSlots) *move_slot(moves, c.scene, i[q] - c.first) = 0u;  // raised by k_gather
+        b[q] = active[q] ? c.cell_base + bucket_of(c, p[q]) : 0xFFFFFFFFu;
+        const uint32_t prev = __shfl_up(b[q], 1, 64);
+        const bool head = lane == 0 || prev != b[q];
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+        first[q] = 63 - __clzll(heads & upto);
+        const uint64_t above = heads & ~upto;
+        next[q] = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    }
+#pragma unroll
+    for (int q = 0; q < kHashItems; ++q) {
+        base[q] = 0u;
+        if (active[q] && lane == first[q]) base[q] = atomicAdd(&counts[b[q]], (uint32_t)(next[q] - first[q]));
+    }
+#pragma unroll
+    for (int q = 0; q < kHashItems; ++q) {
+        const uint32_t bq = __shfl(base[q], first[q], 64);
+        if (active[q]) {
+            keys[i[q]] = b[q];
+            ranks[i[q]] = bq + (uint32_t)(lane - first[q]);
+        }
     }
 }
 
@@ -1469,8 +1489,8 @@ bool valid_block_size(int bs) { return bs == 128 || bs == 256 || bs == 512; }
 
 void launch_hash(const Launch &L, const PoolView &pv, const float4 *P, uint32_t *keys, uint32_t *ranks,
                  uint32_t *counts, uint32_t *flags, uint32_t *moves, uint32_t *nsegs) {
-    LF_GO(L, k_hash, pv.nlaunch / kBlock, kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts, flags, moves,
-          nsegs);
+    LF_GO(L, k_hash, pv.nlaunch / (kBlock * kHashItems), kBlock, L.a, L.b, pv.cs, pv.chunk, P, keys, ranks, counts,
+          flags, moves, nsegs);
 }
 
 uint32_t scan_tiles(uint32_t m) { return (m + kScanTile - 1) / kScanTile; }
