@@ -1161,9 +1161,11 @@ __global__ __launch_bounds__(BS) void k_forces(const Consts *__restrict__ cs, co
 // a pair within h after the move was within h + 2 max|dx| before it, so the
 // masks are complete while no particle moved farther than move_limit
 // (h + 2 move_limit <= h_s with FP32 margins); k_forces flags a farther move
-// in the scene's far-move flag, and then every row is scanned.
+// in the scene's far-move flag, and then every row is scanned.  Registers
+// are capped for five waves per SIMD (four for 512-thread workgroups).
 template <int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_xsph(const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(BS >= 512 ? 4 : 5))) void k_xsph(
+                                             const Consts *__restrict__ cs, const uint32_t *__restrict__ chunk,
                                              uint32_t nslots, const float4 *__restrict__ Ps,
                                              const float4 *__restrict__ Pn, const float4 *__restrict__ Vh,
                                              const float4 *__restrict__ Lw, const uint32_t *__restrict__ start,
