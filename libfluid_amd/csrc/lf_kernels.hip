@@ -21,12 +21,12 @@
 // active span plus an end sentinel.
 //
 // The density pass tests every candidate of the clipped 27-cell stencil (nine
-// contiguous x-row slot ranges) and records, per row, 32/64-bit masks of the
-// slots within h and within a skin radius.  The normals and forces passes walk
-// the exact masks, XSPH the skin masks (when the substep's largest move
-// proves them complete), over packed neighbour data; set bits are visited in
-// slot order, so every pass sums the same neighbours in the same order as a
-// full stencil scan, bit for bit.
+// contiguous x-row slot ranges) and records, per row, a 32/64-bit mask of the
+// slots within a skin radius h_s >= h.  The normals, forces and XSPH passes
+// walk those masks over packed neighbour data (XSPH when the substep's
+// largest move proves them complete) and re-test each pair; set bits are
+// visited in slot order, so every pass sums the same neighbours in the same
+// order as a full stencil scan, bit for bit.
 //
 // SPH here is a gather-bound stencil, not a dense contraction: the kernels
 // are VALU/L1 work over the stencil walk and never use MFMA.
